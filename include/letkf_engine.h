@@ -1,0 +1,171 @@
+/*
+ * letkf_engine.h -- C ABI of the MI355X LETKF local-analysis engine.
+ *
+ * Drop-in for the analysis step of sbPOM-LETKF v2:
+ *   SUBROUTINE da_main(fcst3d,fcst2d,anal3d,anal2d)
+ *   (letkf/run/src/common_da.f90:22-124, called from main.f90:92)
+ * whose other inputs are the module globals of common_setting.f90:168-198
+ * (local grid nij1/lon1/lat1/ri1/rj1/dlon1/dlat1/depth1/phi1 and the sorted
+ * observation tables nobsgrd/obselm/obslon/obslat/obslev/obserr/obsdep/
+ * obshdxf) and the PARAMETERs of common_setting.f90:20-131.
+ *
+ * Every real array has the caller's element type, chosen by
+ * letkf_params_t.real_kind (4 = REAL(4), 8 = REAL(8): the reference's
+ * r_size), and the reference's column-major layout:
+ *   fcst3d/anal3d (nij1,nlev,nmem,nv3d)   fcst2d/anal2d (nij1,nmem,nv2d)
+ *   depth1 (nij1,nlev)   nobsgrd (nlon,nlat) int32   obshdxf (nobs,nmem)
+ * Functions return LETKF_OK (0) or a negative letkf_status_t; they never
+ * abort the process.  letkf_last_error() describes the last failure of the
+ * calling thread.
+ */
+#ifndef LETKF_ENGINE_H
+#define LETKF_ENGINE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum letkf_status {
+  LETKF_OK = 0,
+  LETKF_ERR_ARG = -1,       /* bad argument / inconsistent sizes        */
+  LETKF_ERR_HIP = -2,       /* HIP runtime error (allocation, launch)   */
+  LETKF_ERR_EIG = -3,       /* eigensolve did not converge, or every    */
+                            /* eigenvalue negative (common.f90:1019-1033) */
+  LETKF_ERR_CAPACITY = -4,  /* local obs count beyond the engine limit  */
+  LETKF_ERR_NONFINITE = -5  /* NaN/Inf met where a finite value is needed */
+} letkf_status_t;
+
+/* Arithmetic of the ensemble-space products. */
+typedef enum letkf_precision {
+  LETKF_PREC_FP64 = 0,      /* Gram, weights and transform in FP64       */
+  LETKF_PREC_FP32 = 1,      /* Gram Y^T R^-1 Y on FP32 MFMA, FP64 eig    */
+  LETKF_PREC_BF16X3 = 2     /* Gram from hi/lo BF16 splits on MFMA with  */
+                            /* FP32 accumulation, FP64 eig               */
+} letkf_precision_t;
+
+/* How the per-point ensemble-space problem is solved. */
+typedef enum letkf_form {
+  LETKF_FORM_PLAIN = 0,     /* nmem x nmem eigensolve at every point with */
+                            /* local obs, as letkf_core does             */
+  LETKF_FORM_BATCHED = 1    /* points binned by local-obs count; points  */
+                            /* with nobsl < nmem solve the equivalent    */
+                            /* nobsl x nobsl problem                     */
+} letkf_form_t;
+
+typedef struct letkf_params {
+  int32_t nmem;                 /* K                         (setting:36)  */
+  int32_t nlon, nlat, nlev;     /* global grid               (setting:37-39) */
+  int32_t nij1;                 /* horizontal points of this analysis      */
+  int32_t nv3d, nv2d;           /*                           (setting:40-41) */
+  int32_t iv3d_t, iv3d_s;       /* 1-based T and S slots for round_off    */
+  int32_t id_z_obs;             /* SSH element id, 2567      (setting:59)  */
+  int32_t real_kind;            /* 4 or 8: element type of all real arrays */
+  int32_t roff;                 /* apply round_off           (setting:129) */
+  int32_t precision;            /* letkf_precision_t                       */
+  int32_t form;                 /* letkf_form_t                            */
+  double pi, re;                /*                           (setting:27-28) */
+  double cov_infl_mul;          /* rho                       (setting:71)  */
+  double alpha_rtpp, alpha_rtps;/*                           (setting:74-75) */
+  double sigma_obs, sigma_obsv; /* [m]                       (setting:94-95) */
+  double dist_zero, dist_zerov; /* cut-offs, r_size values   (setting:97-98) */
+  double tmin, tmax, smin, smax;/* round_off bounds          (setting:130-131) */
+} letkf_params_t;
+
+/* Local grid of the analysis (common_setting.f90:168-176). */
+typedef struct letkf_grid {
+  const void* lon1;   /* (nij1) */
+  const void* lat1;
+  const void* ri1;    /* 1-based global i of each point, as a real */
+  const void* rj1;
+  const void* dlon1;  /* grid spacing [m] */
+  const void* dlat1;
+  const void* phi1;   /* bottom depth; 1 marks land (round_off skips it) */
+  const void* depth1; /* (nij1,nlev), negative [m], k = nlev the surface */
+} letkf_grid_t;
+
+/* Observation tables after set_obs (common_setting.f90:182-198): sorted by
+ * (lat cell, lon cell), nobsgrd the cumulative count per cell. */
+typedef struct letkf_obs {
+  int32_t nobs;
+  const int32_t* nobsgrd; /* (nlon,nlat) */
+  const int32_t* elm;     /* (nobs) */
+  const void* lon;        /* (nobs) */
+  const void* lat;
+  const void* lev;        /* depth [m], negative */
+  const void* err;        /* obs error sd */
+  const void* dep;        /* innovation y - H(xf mean) */
+  const void* hdxf;       /* (nobs,nmem) dYf = H(dXf) */
+} letkf_obs_t;
+
+/* Per-call counters and stage times of the last analysis on a handle. */
+typedef struct letkf_stats {
+  int64_t points;          /* nij1*nlev analysed points                */
+  int64_t active_points;   /* points with at least one local obs       */
+  int64_t local_obs;       /* sum of nobsl over points                 */
+  int64_t dual_points;     /* points solved in the nobsl x nobsl form  */
+  int64_t max_nobsl;
+  double ms_total, ms_search, ms_gram, ms_eig, ms_transform;
+  double gram_flops;       /* 2*n*K^2 (or 2*n^2*K dual) summed          */
+  double transform_bytes;  /* fcst read + anal written                 */
+} letkf_stats_t;
+
+typedef struct letkf_handle letkf_handle_t;
+
+/* Fill p with the reference defaults of common_setting.f90 rounded to the
+ * given real_kind (sigma_obs 300 km, sigma_obsv 100 m, rho 1, RTPP 0.9,
+ * RTPS 0, ROFF bounds, cut-offs 2*sqrt(10/3)*sigma). Sizes are left 0. */
+int letkf_params_default(letkf_params_t* p, int32_t real_kind);
+
+/* Create / destroy an engine on the current HIP device.  The handle owns its
+ * device workspaces and one HIP stream; it is not re-entrant, distinct
+ * handles may be used from distinct threads. */
+int letkf_create(const letkf_params_t* p, letkf_handle_t** out);
+int letkf_destroy(letkf_handle_t* h);
+
+/* da_main with HOST arrays (the Fortran drop-in): uploads, analyses,
+ * downloads, synchronous.  Replaces common_da.f90:22-124. */
+int letkf_da_main(letkf_handle_t* h, const letkf_grid_t* grid,
+                  const letkf_obs_t* obs, const void* fcst3d,
+                  const void* fcst2d, void* anal3d, void* anal2d);
+
+/* da_main with DEVICE arrays (every pointer in grid/obs/fcst/anal is device
+ * memory).  Work is queued on `stream` (NULL: the handle's stream) and the
+ * call returns after it completes. */
+int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* grid,
+                         const letkf_obs_t* obs, const void* fcst3d,
+                         const void* fcst2d, void* anal3d, void* anal2d,
+                         void* stream);
+
+/* letkf_core (common_letkf.f90:37-210) batched over nb local problems given
+ * in CSR form, DEVICE arrays: problem b owns rows offs[b]..offs[b+1]-1 of
+ * hdxf (rows of nmem values, row-major), rdiag, rloc, dep; outputs
+ * wvec (nb,nmem) and wmat (nb,nmem,nmem) with wmat[b][i][j] = Wmat(i,j).
+ * A problem with no rows gets letkf_core_noobs (W = sqrt(rho) I, w = 0). */
+int letkf_core_batched(letkf_handle_t* h, int32_t nb, const int64_t* offs,
+                       const void* hdxf, const void* rdiag, const void* rloc,
+                       const void* dep, void* wvec, void* wmat);
+
+/* obs_id_uniform (common.f90:112-168) and the cell sort of set_obs
+ * (common_obs.f90:266-377) on DEVICE arrays: lon/lat the global (nlon,nlat)
+ * grid, olon/olat the nobs unsorted positions.  Writes idx/idy (1-based
+ * cells, 999 outside), perm (sorted position -> input index) and nobsgrd.
+ * *nsorted receives the number of obs inside the grid. */
+int letkf_bin_obs(letkf_handle_t* h, int32_t nlon, int32_t nlat,
+                  const void* lon, const void* lat, int32_t nobs,
+                  const void* olon, const void* olat, int32_t* idx,
+                  int32_t* idy, int32_t* perm, int32_t* nobsgrd,
+                  int32_t* nsorted);
+
+/* AOEI (common_obs.f90:389-400) in place on DEVICE arrays:
+ * err <- sqrt(max(err^2, dep^2 - hxfsprd^2)). */
+int letkf_aoei(letkf_handle_t* h, int32_t nobs, void* err, const void* dep,
+               const void* hxfsprd);
+
+int letkf_get_stats(const letkf_handle_t* h, letkf_stats_t* out);
+const char* letkf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

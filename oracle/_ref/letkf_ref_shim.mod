@@ -1,0 +1,123 @@
+﻿!mod$ v1 sum:2dc0eeab1da00241
+!need$ 0bde2ac47243ead2 i iso_c_binding
+!need$ 2dcd5c82c7cba8f9 n common
+!need$ 7d9124a989da7208 n common_letkf
+!need$ 4ef8e355501b7eb4 n common_setting
+module letkf_ref_shim
+use common_setting,only:r_size
+use common_setting,only:r_dble
+use common_setting,only:nmem
+use common_setting,only:pi
+use common_setting,only:re
+use common,only:distance_2p
+use common,only:mtx_eigen
+use common_letkf,only:letkf_core
+use,intrinsic::iso_c_binding,only:c_associated
+use,intrinsic::iso_c_binding,only:c_funloc
+use,intrinsic::iso_c_binding,only:c_funptr
+use,intrinsic::iso_c_binding,only:c_f_pointer
+use,intrinsic::iso_c_binding,only:c_loc
+use,intrinsic::iso_c_binding,only:c_null_funptr
+use,intrinsic::iso_c_binding,only:c_null_ptr
+use,intrinsic::iso_c_binding,only:c_ptr
+use,intrinsic::iso_c_binding,only:c_sizeof
+use,intrinsic::iso_c_binding,only:operator(==)
+use,intrinsic::iso_c_binding,only:operator(/=)
+use,intrinsic::iso_c_binding,only:c_int8_t
+use,intrinsic::iso_c_binding,only:c_int16_t
+use,intrinsic::iso_c_binding,only:c_int32_t
+use,intrinsic::iso_c_binding,only:c_int64_t
+use,intrinsic::iso_c_binding,only:c_int128_t
+use,intrinsic::iso_c_binding,only:c_int
+use,intrinsic::iso_c_binding,only:c_short
+use,intrinsic::iso_c_binding,only:c_long
+use,intrinsic::iso_c_binding,only:c_long_long
+use,intrinsic::iso_c_binding,only:c_signed_char
+use,intrinsic::iso_c_binding,only:c_size_t
+use,intrinsic::iso_c_binding,only:c_intmax_t
+use,intrinsic::iso_c_binding,only:c_intptr_t
+use,intrinsic::iso_c_binding,only:c_ptrdiff_t
+use,intrinsic::iso_c_binding,only:c_int_least8_t
+use,intrinsic::iso_c_binding,only:c_int_fast8_t
+use,intrinsic::iso_c_binding,only:c_int_least16_t
+use,intrinsic::iso_c_binding,only:c_int_fast16_t
+use,intrinsic::iso_c_binding,only:c_int_least32_t
+use,intrinsic::iso_c_binding,only:c_int_fast32_t
+use,intrinsic::iso_c_binding,only:c_int_least64_t
+use,intrinsic::iso_c_binding,only:c_int_fast64_t
+use,intrinsic::iso_c_binding,only:c_int_least128_t
+use,intrinsic::iso_c_binding,only:c_int_fast128_t
+use,intrinsic::iso_c_binding,only:c_float
+use,intrinsic::iso_c_binding,only:c_double
+use,intrinsic::iso_c_binding,only:c_long_double
+use,intrinsic::iso_c_binding,only:c_float_complex
+use,intrinsic::iso_c_binding,only:c_double_complex
+use,intrinsic::iso_c_binding,only:c_long_double_complex
+use,intrinsic::iso_c_binding,only:c_bool
+use,intrinsic::iso_c_binding,only:c_char
+use,intrinsic::iso_c_binding,only:c_null_char
+use,intrinsic::iso_c_binding,only:c_alert
+use,intrinsic::iso_c_binding,only:c_backspace
+use,intrinsic::iso_c_binding,only:c_form_feed
+use,intrinsic::iso_c_binding,only:c_new_line
+use,intrinsic::iso_c_binding,only:c_carriage_return
+use,intrinsic::iso_c_binding,only:c_horizontal_tab
+use,intrinsic::iso_c_binding,only:c_vertical_tab
+use,intrinsic::iso_c_binding,only:c_float128
+use,intrinsic::iso_c_binding,only:c_float128_complex
+use,intrinsic::iso_c_binding,only:c_uint8_t
+use,intrinsic::iso_c_binding,only:c_uint16_t
+use,intrinsic::iso_c_binding,only:c_uint32_t
+use,intrinsic::iso_c_binding,only:c_uint64_t
+use,intrinsic::iso_c_binding,only:c_uint128_t
+use,intrinsic::iso_c_binding,only:c_unsigned_char
+use,intrinsic::iso_c_binding,only:c_unsigned_short
+use,intrinsic::iso_c_binding,only:c_unsigned
+use,intrinsic::iso_c_binding,only:c_unsigned_long
+use,intrinsic::iso_c_binding,only:c_unsigned_long_long
+use,intrinsic::iso_c_binding,only:c_uintmax_t
+use,intrinsic::iso_c_binding,only:c_uint_fast8_t
+use,intrinsic::iso_c_binding,only:c_uint_fast16_t
+use,intrinsic::iso_c_binding,only:c_uint_fast32_t
+use,intrinsic::iso_c_binding,only:c_uint_fast64_t
+use,intrinsic::iso_c_binding,only:c_uint_fast128_t
+use,intrinsic::iso_c_binding,only:c_uint_least8_t
+use,intrinsic::iso_c_binding,only:c_uint_least16_t
+use,intrinsic::iso_c_binding,only:c_uint_least32_t
+use,intrinsic::iso_c_binding,only:c_uint_least64_t
+use,intrinsic::iso_c_binding,only:c_uint_least128_t
+use,intrinsic::iso_c_binding,only:c_f_procpointer
+contains
+function ref_nmem() bind(c,name="ref_nmem")
+integer(4)::ref_nmem
+end
+function ref_real_kind() bind(c,name="ref_real_kind")
+integer(4)::ref_real_kind
+end
+subroutine ref_letkf_core(nobsl,hdxf,rdiag,rloc,dep,wvec,wmat) bind(c,name="ref_letkf_core")
+integer(4),value::nobsl
+real(4),intent(in)::hdxf(1_8:int(nobsl,kind=8),1_8:128_8)
+real(4),intent(in)::rdiag(1_8:int(nobsl,kind=8))
+real(4),intent(in)::rloc(1_8:int(nobsl,kind=8))
+real(4),intent(in)::dep(1_8:int(nobsl,kind=8))
+real(4),intent(out)::wvec(1_8:128_8)
+real(4),intent(out)::wmat(1_8:128_8,1_8:128_8)
+end
+subroutine ref_mtx_eigen(n,a,eval,evec) bind(c,name="ref_mtx_eigen")
+integer(4),value::n
+real(4),intent(in)::a(1_8:int(n,kind=8),1_8:int(n,kind=8))
+real(8),intent(out)::eval(1_8:int(n,kind=8))
+real(8),intent(out)::evec(1_8:int(n,kind=8),1_8:int(n,kind=8))
+end
+subroutine ref_distance(lona,lata,lonb,latb,dist) bind(c,name="ref_distance")
+real(4),value::lona
+real(4),value::lata
+real(4),value::lonb
+real(4),value::latb
+real(4),intent(out)::dist
+end
+subroutine ref_constants(pi_out,re_out) bind(c,name="ref_constants")
+real(4),intent(out)::pi_out
+real(4),intent(out)::re_out
+end
+end

@@ -1,0 +1,100 @@
+"""ctypes binding of libletkf_engine.so (the C ABI of include/letkf_engine.h).
+
+The engine has no CPU fallback: importing this module without the built
+library, or calling it without a HIP device, raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libletkf_engine.so")
+
+LETKF_OK = 0
+STATUS_NAMES = {0: "LETKF_OK", -1: "LETKF_ERR_ARG", -2: "LETKF_ERR_HIP",
+                -3: "LETKF_ERR_EIG", -4: "LETKF_ERR_CAPACITY",
+                -5: "LETKF_ERR_NONFINITE"}
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+FORM_PLAIN, FORM_BATCHED = 0, 1
+
+# every symbol include/letkf_engine.h declares
+EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
+           "letkf_da_main", "letkf_da_main_device", "letkf_core_batched",
+           "letkf_bin_obs", "letkf_aoei", "letkf_get_stats",
+           "letkf_last_error")
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "nmem", "nlon", "nlat", "nlev", "nij1", "nv3d", "nv2d", "iv3d_t",
+        "iv3d_s", "id_z_obs", "real_kind", "roff", "precision", "form")] + [
+        (n, C.c_double) for n in (
+            "pi", "re", "cov_infl_mul", "alpha_rtpp", "alpha_rtps",
+            "sigma_obs", "sigma_obsv", "dist_zero", "dist_zerov", "tmin",
+            "tmax", "smin", "smax")]
+
+
+class Grid(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "lon1", "lat1", "ri1", "rj1", "dlon1", "dlat1", "phi1", "depth1")]
+
+
+class Obs(C.Structure):
+    _fields_ = [("nobs", C.c_int32), ("nobsgrd", C.c_void_p),
+                ("elm", C.c_void_p)] + [(n, C.c_void_p) for n in (
+                    "lon", "lat", "lev", "err", "dep", "hdxf")]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "points", "active_points", "local_obs", "dual_points", "max_nobsl")] + [
+        (n, C.c_double) for n in (
+            "ms_total", "ms_search", "ms_gram", "ms_eig", "ms_transform",
+            "gram_flops", "transform_bytes")]
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(make -C sbpom_letkf_v2_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.letkf_params_default.argtypes = [C.POINTER(Params), C.c_int32]
+        L.letkf_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+        L.letkf_destroy.argtypes = [vp]
+        L.letkf_da_main.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp]
+        L.letkf_da_main_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp, vp]
+        if hasattr(L, "letkf_core_batched"):
+            L.letkf_core_batched.argtypes = [vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
+        if hasattr(L, "letkf_bin_obs"):
+            L.letkf_bin_obs.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, C.c_int32, vp,
+                                        vp, vp, vp, vp, vp, vp]
+        if hasattr(L, "letkf_aoei"):
+            L.letkf_aoei.argtypes = [vp, C.c_int32, vp, vp, vp]
+        L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.letkf_last_error.restype = C.c_char_p
+        for n in EXPORTS:
+            if n != "letkf_last_error" and hasattr(L, n):
+                getattr(L, n).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != LETKF_OK:
+        raise EngineError(rc, lib().letkf_last_error().decode(errors="replace"))

@@ -1,0 +1,272 @@
+// eig.hip -- batched FP64 symmetric eigensolve of the ensemble-space
+// matrices, replacing mtx_eigen / LAPACK DSYEV
+// (letkf/run/src/common.f90:992-1050): upper triangle read, eigenvalues
+// returned in descending order, negative eigenvalues set to 0 with their
+// vectors ("reconditioning"), status raised when every eigenvalue is
+// negative or an eigenvalue does not converge.
+//
+// Method: Householder reduction to tridiagonal form with accumulated
+// transformations, then implicit-shift QL iterations (Wilkinson & Reinsch,
+// Handbook for Automatic Computation II, 1971).  One wavefront per matrix;
+// the matrix lives in LDS with an odd row pitch (n+1 doubles) so that
+// row-per-lane and column-per-lane sweeps both spread over the banks.  The
+// O(n^2) work of every Householder step and every QL rotation is spread
+// over the 64 lanes (lane r owns rows r, r+64); the scalar QL recurrence is
+// evaluated in every lane.
+#include "letkf_internal.h"
+
+namespace letkf {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Eigen-decomposition of one n x n symmetric matrix held in LDS Z (pitch
+// ld, row-major), d/e scratch of n doubles each.  On return d holds the
+// eigenvalues (unordered) and column j of Z the eigenvector of d[j].
+// Returns false if the QL iteration did not converge.
+__device__ bool eig_wave(int n, double* Z, int ld, double* d, double* e) {
+  const int lane = threadIdx.x & 63;
+#define Z_(r, c) Z[(r) * ld + (c)]
+  // ---- Householder tridiagonalisation (lower triangle) ----
+  for (int i = n - 1; i >= 1; --i) {
+    const int l = i - 1;
+    if (l == 0) {
+      if (lane == 0) {
+        e[i] = Z_(i, 0);
+        d[i] = 0.0;
+      }
+      __syncthreads();
+      continue;
+    }
+    double part = 0.0;
+    for (int c = lane; c <= l; c += 64) part += fabs(Z_(i, c));
+    const double scale = wave_sum(part);
+    if (scale == 0.0) {
+      __syncthreads();
+      if (lane == 0) {
+        e[i] = Z_(i, l);
+        d[i] = 0.0;
+      }
+      __syncthreads();
+      continue;
+    }
+    part = 0.0;
+    for (int c = lane; c <= l; c += 64) {
+      const double v = Z_(i, c) / scale;
+      Z_(i, c) = v;
+      part += v * v;
+    }
+    double h = wave_sum(part);
+    __syncthreads();
+    const double f = Z_(i, l);
+    const double g = f >= 0.0 ? -sqrt(h) : sqrt(h);
+    h -= f * g;
+    __syncthreads();
+    if (lane == 0) {
+      e[i] = scale * g;
+      Z_(i, l) = f - g;
+    }
+    __syncthreads();
+    // p = A u / h into e[0..l]
+    part = 0.0;
+    for (int r = lane; r <= l; r += 64) {
+      Z_(r, i) = Z_(i, r) / h;
+      double acc = 0.0;
+      for (int c = 0; c <= l; ++c) {
+        const double arc = c <= r ? Z_(r, c) : Z_(c, r);
+        acc += arc * Z_(i, c);
+      }
+      const double pr = acc / h;
+      e[r] = pr;
+      part += pr * Z_(i, r);
+    }
+    const double kk = wave_sum(part) / (h + h);
+    __syncthreads();
+    for (int r = lane; r <= l; r += 64) e[r] = e[r] - kk * Z_(i, r);
+    __syncthreads();
+    for (int r = lane; r <= l; r += 64) {
+      const double ur = Z_(i, r), qr = e[r];
+      for (int c = 0; c <= r; ++c) Z_(r, c) -= ur * e[c] + qr * Z_(i, c);
+    }
+    if (lane == 0) d[i] = h;
+    __syncthreads();
+  }
+  if (lane == 0) {
+    d[0] = 0.0;
+    e[0] = 0.0;
+  }
+  __syncthreads();
+  // ---- accumulate Q ----
+  for (int i = 0; i < n; ++i) {
+    const int l = i - 1;
+    if (d[i] != 0.0) {
+      for (int c = lane; c <= l; c += 64) {
+        double g = 0.0;
+        for (int r = 0; r <= l; ++r) g += Z_(i, r) * Z_(r, c);
+        for (int r = 0; r <= l; ++r) Z_(r, c) -= g * Z_(r, i);
+      }
+    }
+    __syncthreads();
+    if (lane == 0) d[i] = Z_(i, i);
+    __syncthreads();
+    for (int c = lane; c <= l; c += 64) {
+      Z_(c, i) = 0.0;
+      Z_(i, c) = 0.0;
+    }
+    if (lane == 0) Z_(i, i) = 1.0;
+    __syncthreads();
+  }
+  // ---- implicit QL ----
+  for (int q = lane + 1; q < n; q += 64) e[q - 1] = e[q];
+  __syncthreads();
+  if (lane == 0) e[n - 1] = 0.0;
+  __syncthreads();
+  bool ok = true;
+  for (int l = 0; l < n; ++l) {
+    int sweeps = 0;
+    for (;;) {
+      int m;
+      for (m = l; m < n - 1; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (m == l) break;
+      if (++sweeps > 60) {
+        ok = false;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      double s = 1.0, c = 1.0, p = 0.0;
+      bool deflated = false;
+      // column i+1 of the row carried in a register across the sweep
+      double carry0 = lane < n ? Z_(lane, m) : 0.0;
+      double carry1 = lane + 64 < n ? Z_(lane + 64, m) : 0.0;
+      for (int i = m - 1; i >= l; --i) {
+        const double ei = e[i], f = s * ei, b = c * ei;
+        r = hypot(f, g);
+        __syncthreads();
+        if (lane == 0) e[i + 1] = r;
+        if (r == 0.0) {
+          if (lane == 0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+          }
+          if (lane < n) Z_(lane, i + 1) = carry0;
+          if (lane + 64 < n) Z_(lane + 64, i + 1) = carry1;
+          deflated = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        __syncthreads();
+        if (lane == 0) d[i + 1] = g + p;
+        g = c * r - b;
+        if (lane < n) {
+          const double zi = Z_(lane, i);
+          Z_(lane, i + 1) = s * zi + c * carry0;
+          carry0 = c * zi - s * carry0;
+        }
+        if (lane + 64 < n) {
+          const double zi = Z_(lane + 64, i);
+          Z_(lane + 64, i + 1) = s * zi + c * carry1;
+          carry1 = c * zi - s * carry1;
+        }
+      }
+      if (!ok) break;
+      if (deflated) {
+        __syncthreads();
+        continue;
+      }
+      if (lane < n) Z_(lane, l) = carry0;
+      if (lane + 64 < n) Z_(lane + 64, l) = carry1;
+      __syncthreads();
+      if (lane == 0) {
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+      __syncthreads();
+    }
+    if (!ok) break;
+  }
+  __syncthreads();
+#undef Z_
+  return ok;
+}
+
+// One matrix per 64-lane workgroup.  Dynamic LDS: n*(n+1) + 2n doubles.
+__global__ void __launch_bounds__(64) k_eig_fp64(int n, const double* __restrict__ G, double* __restrict__ lam,
+                                                 double* __restrict__ V, int* __restrict__ status) {
+  extern __shared__ double lds[];
+  const int ld = n + 1;
+  double* Z = lds;
+  double* d = Z + n * ld;
+  double* e = d + n;
+  const int lane = threadIdx.x;
+  const double* Gb = G + (long long)blockIdx.x * n * n;
+  // symmetric copy from the upper triangle (DSYEV('V','U') semantics)
+  for (int q = lane; q < n * n; q += 64) {
+    const int r = q / n, c = q - r * n;
+    Z[r * ld + c] = r <= c ? Gb[r * n + c] : Gb[c * n + r];
+  }
+  __syncthreads();
+  bool ok;
+  if (n == 1) {
+    if (lane == 0) {
+      d[0] = Z[0];
+      Z[0] = 1.0;
+    }
+    __syncthreads();
+    ok = true;
+  } else {
+    ok = eig_wave(n, Z, ld, d, e);
+  }
+  // descending order; ties keep the reverse of DSYEV's ascending order
+  double* lb = lam + (long long)blockIdx.x * n;
+  double* Vb = V + (long long)blockIdx.x * n * n;
+  double dmax = -INFINITY;
+  bool finite = true;
+  for (int j = 0; j < n; ++j) {
+    dmax = fmax(dmax, d[j]);
+    finite = finite && isfinite(d[j]);
+  }
+  for (int j = lane; j < n && finite; j += 64) {
+    const double dj = d[j];
+    int rank = 0;
+    for (int c = 0; c < n; ++c) rank += (d[c] > dj) || (d[c] == dj && c > j);
+    const bool neg = dj < 0.0;
+    lb[rank] = neg ? 0.0 : dj;
+    for (int r = 0; r < n; ++r) Vb[r * n + rank] = neg ? 0.0 : Z[r * ld + j];
+  }
+  if (lane == 0) {
+    int st = 0;
+    if (!ok) st |= ST_EIG_NOCONV;
+    if (dmax < 0.0) st |= ST_EIG_NEG;
+    if (!finite) st |= ST_NONFINITE;
+    if (st) atomicOr(status, st);
+  }
+}
+
+hipError_t launch_eig_fp64(int K, int na, const double* G, double* work, double* lam, double* V, int* status,
+                           hipStream_t s) {
+  (void)work;
+  if (na <= 0) return hipSuccess;
+  const size_t lds = sizeof(double) * ((size_t)K * (K + 1) + 2 * (size_t)K);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_eig_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_eig_fp64, dim3(na), dim3(64), lds, s, K, G, lam, V, status);
+  return hipGetLastError();
+}
+
+}  // namespace letkf

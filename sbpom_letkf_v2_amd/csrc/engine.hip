@@ -1,0 +1,424 @@
+// engine.hip -- host side of the C ABI in include/letkf_engine.h: argument
+// checks, device workspaces, and the per-analysis pipeline
+//   search count -> point scans -> per chunk of points:
+//   search fill -> Gram -> eigensolve -> transform (+RTPP/RTPS/round_off)
+// that replaces da_main (letkf/run/src/common_da.f90:22-124).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "letkf_internal.h"
+
+using namespace letkf;
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      return set_err(LETKF_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),     \
+                     __FILE__, __LINE__);                                                      \
+  } while (0)
+
+namespace {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (ptr) {
+      hipError_t e = hipFree(ptr);
+      if (e != hipSuccess) return e;
+      ptr = nullptr;
+      bytes = 0;
+    }
+    want = want < 256 ? 256 : want;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename X>
+  X* as() const {
+    return (X*)ptr;
+  }
+};
+
+}  // namespace
+
+struct letkf_handle {
+  letkf_params_t prm;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf nloc, loff, actidx, act_p, scan_tmp, yT, lobs, lrloc, G, lam, V, bvec, status, bounds;
+  // staging for the host-pointer entry point
+  DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
+  letkf_stats_t stats{};
+};
+
+static int check_params(const letkf_params_t* p) {
+  if (!p) return set_err(LETKF_ERR_ARG, "params is NULL");
+  if (p->real_kind != 4 && p->real_kind != 8)
+    return set_err(LETKF_ERR_ARG, "real_kind must be 4 or 8 (got %d)", p->real_kind);
+  if (p->nmem < 2 || p->nmem > kMaxK)
+    return set_err(LETKF_ERR_ARG, "nmem must be in [2,%d] (got %d)", kMaxK, p->nmem);
+  if (p->nlon < 2 || p->nlat < 2 || p->nlev < 1 || p->nij1 < 1)
+    return set_err(LETKF_ERR_ARG, "bad grid sizes nlon=%d nlat=%d nlev=%d nij1=%d", p->nlon, p->nlat,
+                   p->nlev, p->nij1);
+  if (p->nv3d < 0 || p->nv2d < 0 || p->nv3d + p->nv2d > 8 || p->nv3d + p->nv2d < 1)
+    return set_err(LETKF_ERR_ARG, "nv3d+nv2d must be in [1,8] (got %d+%d)", p->nv3d, p->nv2d);
+  if (!(p->cov_infl_mul > 0.0)) return set_err(LETKF_ERR_ARG, "cov_infl_mul must be > 0");
+  if (p->precision != LETKF_PREC_FP64 && p->precision != LETKF_PREC_FP32 && p->precision != LETKF_PREC_BF16X3)
+    return set_err(LETKF_ERR_ARG, "unknown precision %d", p->precision);
+  if (p->form != LETKF_FORM_PLAIN && p->form != LETKF_FORM_BATCHED)
+    return set_err(LETKF_ERR_ARG, "unknown form %d", p->form);
+  if ((long long)p->nij1 * p->nlev > (1LL << 31) - 2)
+    return set_err(LETKF_ERR_ARG, "nij1*nlev beyond int32 range");
+  return LETKF_OK;
+}
+
+extern "C" int letkf_params_default(letkf_params_t* p, int32_t real_kind) {
+  if (!p) return set_err(LETKF_ERR_ARG, "params is NULL");
+  if (real_kind != 4 && real_kind != 8) return set_err(LETKF_ERR_ARG, "real_kind must be 4 or 8");
+  memset(p, 0, sizeof *p);
+  auto R = [&](double x) { return real_kind == 4 ? (double)(float)x : x; };
+  p->real_kind = real_kind;
+  p->iv3d_t = 3;
+  p->iv3d_s = 4;
+  p->id_z_obs = 2567;
+  p->roff = 1;
+  p->precision = LETKF_PREC_FP64;
+  p->form = LETKF_FORM_PLAIN;
+  p->pi = R(4.0 * atan(1.0));
+  p->re = R(6371.3e3);
+  p->cov_infl_mul = R(1.0);
+  p->alpha_rtpp = R(0.9);
+  p->alpha_rtps = R(0.0);
+  p->sigma_obs = R(300.0e3);
+  p->sigma_obsv = R(100.0);
+  // REAL(2*sqrt(10/3), r_size) * sigma, product in r_size
+  if (real_kind == 4) {
+    p->dist_zero = (double)((float)(2.0 * sqrt(10.0 / 3.0)) * (float)300.0e3);
+    p->dist_zerov = (double)((float)(2.0 * sqrt(10.0 / 3.0)) * (float)100.0);
+  } else {
+    p->dist_zero = 2.0 * sqrt(10.0 / 3.0) * 300.0e3;
+    p->dist_zerov = 2.0 * sqrt(10.0 / 3.0) * 100.0;
+  }
+  p->tmin = R(-1.8);
+  p->tmax = R(50.0);
+  p->smin = R(25.0);
+  p->smax = R(50.0);
+  return LETKF_OK;
+}
+
+extern "C" int letkf_create(const letkf_params_t* p, letkf_handle_t** out) {
+  if (!out) return set_err(LETKF_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  letkf_handle* h = new letkf_handle();
+  h->prm = *p;
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = h->status.ensure(sizeof(int) * 4);
+  if (e != hipSuccess) {
+    delete h;
+    return set_err(LETKF_ERR_HIP, "letkf_create: %s", hipGetErrorString(e));
+  }
+  *out = h;
+  return LETKF_OK;
+}
+
+extern "C" int letkf_destroy(letkf_handle_t* h) {
+  if (!h) return LETKF_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  DevBuf* all[] = {&h->nloc, &h->loff,  &h->actidx, &h->act_p, &h->scan_tmp, &h->yT,   &h->lobs,
+                   &h->lrloc, &h->G,    &h->lam,    &h->V,     &h->bvec,     &h->status, &h->bounds,
+                   &h->h_grid, &h->h_obs, &h->h_f3,  &h->h_f2,  &h->h_a3,     &h->h_a2};
+  for (DevBuf* b : all) b->release();
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return LETKF_OK;
+}
+
+extern "C" const char* letkf_last_error(void) { return g_err; }
+
+extern "C" int letkf_get_stats(const letkf_handle_t* h, letkf_stats_t* out) {
+  if (!h || !out) return set_err(LETKF_ERR_ARG, "NULL argument");
+  *out = h->stats;
+  return LETKF_OK;
+}
+
+static KParams make_kparams(const letkf_params_t& p, int nobs) {
+  KParams kp;
+  kp.K = p.nmem;
+  kp.nlon = p.nlon;
+  kp.nlat = p.nlat;
+  kp.nlev = p.nlev;
+  kp.nij1 = p.nij1;
+  kp.nv3d = p.nv3d;
+  kp.nv2d = p.nv2d;
+  kp.iv3d_t = p.iv3d_t;
+  kp.iv3d_s = p.iv3d_s;
+  kp.id_z_obs = p.id_z_obs;
+  kp.roff = p.roff;
+  kp.nobs = nobs;
+  kp.P = (long long)p.nij1 * p.nlev;
+  kp.pi = p.pi;
+  kp.re = p.re;
+  kp.rho = p.cov_infl_mul;
+  kp.alpha_rtpp = p.alpha_rtpp;
+  kp.alpha_rtps = p.alpha_rtps;
+  kp.sigma_obs = p.sigma_obs;
+  kp.sigma_obsv = p.sigma_obsv;
+  kp.dist_zero = p.dist_zero;
+  kp.dist_zerov = p.dist_zerov;
+  kp.tmin = p.tmin;
+  kp.tmax = p.tmax;
+  kp.smin = p.smin;
+  kp.smax = p.smax;
+  return kp;
+}
+
+// Collect loff/actidx at chunk boundaries: out[2c] = loff[b[c]], out[2c+1] = actidx[b[c]].
+__global__ void k_gather_bounds(int nb, const long long* pts, const int64_t* loff, const int32_t* actidx,
+                                long long* out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nb) {
+    out[2 * c] = loff[pts[c]];
+    out[2 * c + 1] = actidx[pts[c]];
+  }
+}
+
+template <typename T>
+static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2, T* a3,
+                      T* a2, hipStream_t s) {
+  const letkf_params_t& p = h->prm;
+  const int K = p.nmem;
+  KParams kp = make_kparams(p, o->nobs);
+  const long long P = kp.P;
+  const size_t n3 = (size_t)P * K * p.nv3d, n2 = (size_t)p.nij1 * K * p.nv2d;
+  letkf_stats_t st{};
+  st.points = P;
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  HIP_TRY(hipEventRecord(ev0, s));
+  if (o->nobs == 0) {
+    // common_da.f90:57-61: analysis = forecast, no inflation, no round_off
+    HIP_TRY(launch_copy<T>(f3, a3, n3, s));
+    HIP_TRY(launch_copy<T>(f2, a2, n2, s));
+    HIP_TRY(hipEventRecord(ev1, s));
+    HIP_TRY(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    st.ms_total = ms;
+    h->stats = st;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return LETKF_OK;
+  }
+  Tables<T> tb;
+  tb.lon1 = (const T*)g->lon1;
+  tb.lat1 = (const T*)g->lat1;
+  tb.ri1 = (const T*)g->ri1;
+  tb.rj1 = (const T*)g->rj1;
+  tb.dlon1 = (const T*)g->dlon1;
+  tb.dlat1 = (const T*)g->dlat1;
+  tb.phi1 = (const T*)g->phi1;
+  tb.depth1 = (const T*)g->depth1;
+  tb.nobsgrd = o->nobsgrd;
+  tb.elm = o->elm;
+  tb.olon = (const T*)o->lon;
+  tb.olat = (const T*)o->lat;
+  tb.olev = (const T*)o->lev;
+  tb.oerr = (const T*)o->err;
+  tb.odep = (const T*)o->dep;
+
+  HIP_TRY(h->nloc.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->loff.ensure(sizeof(int64_t) * (P + 1)));
+  HIP_TRY(h->actidx.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->act_p.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->scan_tmp.ensure(scan_tmp_bytes(P)));
+  HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
+  HIP_TRY(hipMemsetAsync(h->status.ptr, 0, sizeof(int) * 4, s));
+
+  // obs-major FP64 copy of obshdxf
+  HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
+  // count_nobsl for every point, then the scans
+  HIP_TRY(launch_search_count<T>(kp, tb, h->nloc.as<int32_t>(), s));
+  HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
+                             h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
+
+  // chunks of consecutive points sized so that the per-point K x K
+  // workspaces (G, V) stay within ~2 GiB each
+  long long PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
+  PC = std::max(1024LL, std::min(PC, 65536LL));
+  const long long nch = (P + PC - 1) / PC;
+  std::vector<long long> bpts(nch + 1);
+  for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
+  HIP_TRY(h->bounds.ensure(sizeof(long long) * 3 * (nch + 1)));
+  long long* d_pts = h->bounds.as<long long>();
+  long long* d_out = d_pts + (nch + 1);
+  HIP_TRY(hipMemcpyAsync(d_pts, bpts.data(), sizeof(long long) * (nch + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_gather_bounds, dim3((unsigned)((nch + 1 + 255) / 256)), dim3(256), 0, s, (int)(nch + 1),
+                     d_pts, h->loff.as<int64_t>(), h->actidx.as<int32_t>(), d_out);
+  HIP_TRY(hipGetLastError());
+  std::vector<long long> bv(2 * (nch + 1));
+  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * 2 * (nch + 1), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  long long maxL = 0, maxA = 0;
+  for (long long c = 0; c < nch; ++c) {
+    maxL = std::max(maxL, bv[2 * (c + 1)] - bv[2 * c]);
+    maxA = std::max(maxA, bv[2 * (c + 1) + 1] - bv[2 * c + 1]);
+  }
+  st.local_obs = bv[2 * nch];
+  st.active_points = bv[2 * nch + 1];
+  HIP_TRY(h->lobs.ensure(sizeof(int32_t) * (maxL + 1)));
+  HIP_TRY(h->lrloc.ensure(sizeof(double) * (maxL + 1)));
+  HIP_TRY(h->G.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
+  HIP_TRY(h->V.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
+  HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+  HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+
+  for (long long c = 0; c < nch; ++c) {
+    const long long p0 = bpts[c], p1 = bpts[c + 1];
+    const long long lbase = bv[2 * c];
+    const int a0 = (int)bv[2 * c + 1];
+    const int na = (int)(bv[2 * (c + 1) + 1] - a0);
+    if (na > 0) {
+      HIP_TRY(launch_search_fill<T>(kp, tb, p0, p1, h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
+                                    h->lrloc.as<double>(), s));
+      HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
+                                  lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na, h->G.as<double>(),
+                                  h->bvec.as<double>(), s));
+      HIP_TRY(launch_eig_fp64(K, na, h->G.as<double>(), nullptr, h->lam.as<double>(), h->V.as<double>(),
+                              h->status.as<int>(), s));
+    }
+    HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,
+                                h->lam.as<double>(), h->V.as<double>(), h->bvec.as<double>(), f3, f2, a3, a2, s));
+  }
+  HIP_TRY(hipEventRecord(ev1, s));
+  int status = 0;
+  HIP_TRY(hipMemcpyAsync(&status, h->status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  st.ms_total = ms;
+  st.gram_flops = 2.0 * (double)st.local_obs * K * K;
+  st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
+  h->stats = st;
+  if (status & ST_NONFINITE) return set_err(LETKF_ERR_NONFINITE, "non-finite ensemble-space matrix");
+  if (status & ST_EIG_NOCONV) return set_err(LETKF_ERR_EIG, "eigensolve did not converge");
+  if (status & ST_EIG_NEG) return set_err(LETKF_ERR_EIG, "all eigenvalues negative");
+  return LETKF_OK;
+}
+
+static int check_call(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
+                      const void* f2, void* a3, void* a2) {
+  if (!h) return set_err(LETKF_ERR_ARG, "handle is NULL");
+  if (!g || !o) return set_err(LETKF_ERR_ARG, "grid/obs is NULL");
+  if (o->nobs < 0) return set_err(LETKF_ERR_ARG, "nobs < 0");
+  const letkf_params_t& p = h->prm;
+  if ((p.nv3d && (!f3 || !a3)) || (p.nv2d && (!f2 || !a2)))
+    return set_err(LETKF_ERR_ARG, "fcst/anal array is NULL");
+  if (o->nobs > 0) {
+    const void* need[] = {g->lon1, g->lat1, g->ri1, g->rj1, g->dlon1, g->dlat1, g->phi1, g->depth1,
+                          o->nobsgrd, o->elm, o->lon, o->lat, o->lev, o->err, o->dep, o->hdxf};
+    for (const void* q : need)
+      if (!q) return set_err(LETKF_ERR_ARG, "a grid/obs array is NULL");
+  }
+  return LETKF_OK;
+}
+
+extern "C" int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o,
+                                    const void* f3, const void* f2, void* a3, void* a2, void* stream) {
+  int rc = check_call(h, g, o, f3, f2, a3, a2);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  if (h->prm.real_kind == 8)
+    return run_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s);
+  return run_device<float>(h, g, o, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s);
+}
+
+extern "C" int letkf_da_main(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
+                             const void* f2, void* a3, void* a2) {
+  int rc = check_call(h, g, o, f3, f2, a3, a2);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(h->device));
+  const letkf_params_t& p = h->prm;
+  const size_t R = p.real_kind;
+  const size_t nij = p.nij1, nlev = p.nlev, K = p.nmem, nobs = o->nobs;
+  const size_t n3 = nij * nlev * K * p.nv3d, n2 = nij * K * p.nv2d;
+  hipStream_t s = h->stream;
+  // grid arrays: 7 x nij1 + depth1 (nij1 x nlev)
+  const size_t gbytes = R * (7 * nij + nij * nlev);
+  const size_t obytes = sizeof(int32_t) * ((size_t)p.nlon * p.nlat + nobs) + R * (5 * nobs + nobs * K);
+  HIP_TRY(h->h_grid.ensure(gbytes));
+  HIP_TRY(h->h_obs.ensure(obytes + 64));
+  HIP_TRY(h->h_f3.ensure(R * n3));
+  HIP_TRY(h->h_f2.ensure(R * n2));
+  HIP_TRY(h->h_a3.ensure(R * n3));
+  HIP_TRY(h->h_a2.ensure(R * n2));
+  char* gp = h->h_grid.as<char>();
+  letkf_grid_t dg;
+  const void* gsrc[8] = {g->lon1, g->lat1, g->ri1, g->rj1, g->dlon1, g->dlat1, g->phi1, g->depth1};
+  const void** gdst[8] = {&dg.lon1, &dg.lat1, &dg.ri1, &dg.rj1, &dg.dlon1, &dg.dlat1, &dg.phi1, &dg.depth1};
+  for (int q = 0; q < 8; ++q) {
+    const size_t b = R * (q == 7 ? nij * nlev : nij);
+    *gdst[q] = gp;
+    if (gsrc[q]) HIP_TRY(hipMemcpyAsync(gp, gsrc[q], b, hipMemcpyHostToDevice, s));
+    gp += b;
+  }
+  char* op = h->h_obs.as<char>();
+  letkf_obs_t dobs;
+  dobs.nobs = o->nobs;
+  auto put = [&](const void* src, size_t b) -> const void* {
+    const void* dst = op;
+    if (src && b) {
+      hipError_t e = hipMemcpyAsync(op, src, b, hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) return nullptr;
+    }
+    op += (b + 7) & ~(size_t)7;
+    return dst;
+  };
+  dobs.nobsgrd = (const int32_t*)put(o->nobsgrd, sizeof(int32_t) * p.nlon * p.nlat);
+  dobs.elm = (const int32_t*)put(o->elm, sizeof(int32_t) * nobs);
+  dobs.lon = put(o->lon, R * nobs);
+  dobs.lat = put(o->lat, R * nobs);
+  dobs.lev = put(o->lev, R * nobs);
+  dobs.err = put(o->err, R * nobs);
+  dobs.dep = put(o->dep, R * nobs);
+  dobs.hdxf = put(o->hdxf, R * nobs * K);
+  if (n3) HIP_TRY(hipMemcpyAsync(h->h_f3.ptr, f3, R * n3, hipMemcpyHostToDevice, s));
+  if (n2) HIP_TRY(hipMemcpyAsync(h->h_f2.ptr, f2, R * n2, hipMemcpyHostToDevice, s));
+  rc = letkf_da_main_device(h, &dg, &dobs, h->h_f3.ptr, h->h_f2.ptr, h->h_a3.ptr, h->h_a2.ptr, s);
+  if (rc) return rc;
+  if (n3) HIP_TRY(hipMemcpyAsync(a3, h->h_a3.ptr, R * n3, hipMemcpyDeviceToHost, s));
+  if (n2) HIP_TRY(hipMemcpyAsync(a2, h->h_a2.ptr, R * n2, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return LETKF_OK;
+}

@@ -1,0 +1,90 @@
+// letkf_internal.h -- device-side types and launcher declarations shared by
+// the engine's HIP translation units.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/letkf_engine.h"
+
+namespace letkf {
+
+// Scalars of one analysis, by value into every kernel.
+struct KParams {
+  int K, nlon, nlat, nlev, nij1, nv3d, nv2d, iv3d_t, iv3d_s, id_z_obs, roff;
+  int nobs;
+  long long P;  // nij1 * nlev
+  double pi, re, rho, alpha_rtpp, alpha_rtps, sigma_obs, sigma_obsv;
+  double dist_zero, dist_zerov, tmin, tmax, smin, smax;
+};
+
+// Local grid + sorted obs tables, typed by the caller's REAL.
+template <typename T>
+struct Tables {
+  const T *lon1, *lat1, *ri1, *rj1, *dlon1, *dlat1, *phi1, *depth1;
+  const int32_t *nobsgrd, *elm;
+  const T *olon, *olat, *olev, *oerr, *odep;
+};
+
+// Per-analysis device arrays over all P points.
+struct PointArrays {
+  int32_t* nloc;    // [P]   local obs count (nobsl)
+  int64_t* loff;    // [P+1] exclusive scan of nloc
+  int32_t* actidx;  // [P+1] exclusive scan of (nloc > 0)
+  int32_t* act_p;   // [nactive] point of each active slot
+};
+
+// Status bits raised by kernels (atomicOr into a device word).
+enum : int {
+  ST_EIG_NOCONV = 1,
+  ST_EIG_NEG = 2,
+  ST_CAPACITY = 4,
+  ST_NONFINITE = 8
+};
+
+constexpr int kMaxK = 128;  // largest ensemble handled on chip
+
+// ---- launchers (each defined in the .hip that holds its kernel) ----
+template <typename T>
+hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb,
+                               int32_t* nloc, hipStream_t s);
+template <typename T>
+hipError_t launch_search_fill(const KParams& kp, const Tables<T>& tb,
+                              long long p0, long long p1, const int64_t* loff,
+                              int32_t* lobs, double* lrloc, hipStream_t s);
+
+hipError_t launch_scan_points(long long P, const int32_t* nloc, int64_t* loff,
+                              int32_t* actidx, int32_t* act_p, void* tmp,
+                              size_t tmp_bytes, hipStream_t s);
+size_t scan_tmp_bytes(long long P);
+
+template <typename T>
+hipError_t launch_transpose_obs(int nobs, int K, const T* hdxf, double* yT,
+                                hipStream_t s);
+
+// Gram G = Y^T diag(rloc/rdiag) Y + (K-1)/rho I and b = Y^T diag(rloc/rdiag) d
+// for active slots [a0, a0+na) -> G point-major [na][K][K], bvec [na][K].
+template <typename T>
+hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
+                            const double* yT, const int32_t* act_p,
+                            const int64_t* loff, long long lbase,
+                            const int32_t* lobs, const double* lrloc, int a0,
+                            int na, double* G, double* bvec, hipStream_t s);
+
+// Batched symmetric eigensolve: G [na][K][K] (upper triangle read) ->
+// lam [na][K] descending, V [na][K][K] (V[r][j] = component r of vector j).
+hipError_t launch_eig_fp64(int K, int na, const double* G, double* work,
+                           double* lam, double* V, int* status, hipStream_t s);
+
+// Weights + ensemble transform + RTPP/RTPS + round_off for points [p0,p1).
+template <typename T>
+hipError_t launch_transform(const KParams& kp, const Tables<T>& tb,
+                            long long p0, long long p1, const int32_t* nloc,
+                            const int32_t* actidx, int a0, const double* lam,
+                            const double* V, const double* bvec,
+                            const T* fcst3d, const T* fcst2d, T* anal3d,
+                            T* anal2d, hipStream_t s);
+
+template <typename T>
+hipError_t launch_copy(const T* src, T* dst, size_t n, hipStream_t s);
+
+}  // namespace letkf

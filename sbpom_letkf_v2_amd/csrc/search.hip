@@ -1,0 +1,322 @@
+// search.hip -- local-observation search of da_main (count_nobsl /
+// obs_local / obs_local_sub, letkf/run/src/common_da.f90:133-348), the
+// point scans that turn local counts into list offsets, and the obs-major
+// transpose of obshdxf.
+//
+// The search keeps the reference's r_size semantics: the box edges use
+// NINT (round half away from zero) of r_size values, distance_2p
+// (common.f90:79-101) stores cos(d) in r_size before ACOS, and rloc is
+// rounded to r_size.  T is the caller's REAL.
+#include "letkf_internal.h"
+
+namespace letkf {
+
+template <typename T> __device__ __forceinline__ int nint_r(T x);
+template <> __device__ __forceinline__ int nint_r<double>(double x) { return (int)round(x); }
+template <> __device__ __forceinline__ int nint_r<float>(float x) { return (int)roundf(x); }
+
+template <typename T> __device__ __forceinline__ T acos_r(T x);
+template <> __device__ __forceinline__ double acos_r<double>(double x) { return acos(x); }
+template <> __device__ __forceinline__ float acos_r<float>(float x) { return acosf(x); }
+
+template <typename T> __device__ __forceinline__ T abs_r(T x) { return x < (T)0 ? -x : x; }
+
+// Walks the search box of point (i,k) in the reference's order and calls
+// visit(io, dist, dlev) for each accepted obs io (0-based).
+template <typename T, typename Visit>
+__device__ __forceinline__ int search_point(const KParams& kp, const Tables<T>& tb,
+                                            int i, int k, Visit&& visit) {
+#pragma clang fp contract(off)
+  const T dz = (T)kp.dist_zero, dzv = (T)kp.dist_zerov;
+  const T pi = (T)kp.pi, re = (T)kp.re;
+  const T rdep = tb.depth1[i + (long long)kp.nij1 * k];
+  const T hx = dz / tb.dlon1[i], hy = dz / tb.dlat1[i];
+  int imin = nint_r<T>(tb.ri1[i] - hx);
+  int imax = nint_r<T>(tb.ri1[i] + hx);
+  int jmin = nint_r<T>(tb.rj1[i] - hy);
+  int jmax = nint_r<T>(tb.rj1[i] + hy);
+  imin = imin < 1 ? 1 : imin;
+  imax = imax > kp.nlon ? kp.nlon : imax;
+  jmin = jmin < 1 ? 1 : jmin;
+  jmax = jmax > kp.nlat ? kp.nlat : jmax;
+  const T lonb = tb.lon1[i];
+  const double bb = (double)(T)(pi * tb.lat1[i]) / 180.0;
+  const double sinb = sin(bb), cosb = cos(bb);
+  const bool below_surface = k < kp.nlev - 1;
+  int count = 0;
+  for (int j = jmin; j <= jmax; ++j) {
+    long long ib;
+    if (imin > 1)
+      ib = (long long)tb.nobsgrd[(imin - 2) + (long long)kp.nlon * (j - 1)] + 1;
+    else if (j > 1)
+      ib = (long long)tb.nobsgrd[(kp.nlon - 1) + (long long)kp.nlon * (j - 2)] + 1;
+    else
+      ib = 1;
+    const long long ie = tb.nobsgrd[(imax - 1) + (long long)kp.nlon * (j - 1)];
+    for (long long ip = ib; ip <= ie; ++ip) {
+      const int io = (int)(ip - 1);
+      const int el = tb.elm[io];
+      T dlev;
+      if (el == kp.id_z_obs && below_surface)
+        dlev = abs_r<T>(rdep);
+      else if (el != kp.id_z_obs)
+        dlev = abs_r<T>(tb.olev[io] - rdep);
+      else
+        dlev = (T)0;
+      if (dzv < dlev) continue;
+      const double aa = (double)(T)(pi * tb.olat[io]) / 180.0;
+      const double cc = (double)(T)(pi * (T)(lonb - tb.olon[io])) / 180.0;
+      const double prod1 = sin(aa) * sinb;
+      const double prod2 = cos(aa) * cosb;
+      T cosd = (T)(prod1 + prod2 * cos(cc));
+      if ((double)cosd > 1.0) cosd = (T)1;
+      if ((double)cosd < -1.0) cosd = (T)-1;
+      const T dist = re * acos_r<T>(cosd);
+      if (dz < dist) continue;
+      visit(count, io, dist, dlev);
+      ++count;
+    }
+  }
+  return count;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_search_count(KParams kp, Tables<T> tb, int32_t* nloc) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < kp.P; p += stride) {
+    const int k = (int)(p / kp.nij1), i = (int)(p - (long long)k * kp.nij1);
+    nloc[p] = search_point<T>(kp, tb, i, k, [](int, int, T, T) {});
+  }
+}
+
+// obs_local's gather half: the obs index and rloc of every accepted obs,
+// rloc = exp(-0.5*((dist/sigma_obs)^2 + (dlev/sigma_obsv)^2)) rounded to
+// r_size (common_da.f90:282-287).
+template <typename T>
+__global__ void __launch_bounds__(256) k_search_fill(KParams kp, Tables<T> tb, long long p0, long long p1,
+                                                     const int64_t* loff, int32_t* lobs, double* lrloc) {
+  const long long base = loff[p0];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const T so = (T)kp.sigma_obs, sv = (T)kp.sigma_obsv;
+  const bool vloc = !(kp.sigma_obsv <= 0.0);
+  for (long long p = p0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; p < p1; p += stride) {
+    const long long o = loff[p] - base;
+    if (loff[p + 1] == loff[p]) continue;
+    const int k = (int)(p / kp.nij1), i = (int)(p - (long long)k * kp.nij1);
+    search_point<T>(kp, tb, i, k, [&](int l, int io, T dist, T dlev) {
+#pragma clang fp contract(off)
+      T q = (dist / so) * (dist / so);
+      if (vloc) q = q + (dlev / sv) * (dlev / sv);
+      lobs[o + l] = io;
+      lrloc[o + l] = (double)(T)exp(-0.5 * (double)q);
+    });
+  }
+}
+
+template <typename T>
+hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb, int32_t* nloc, hipStream_t s) {
+  long long blocks = (kp.P + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_search_count<T>, dim3((unsigned)blocks), dim3(256), 0, s, kp, tb, nloc);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_search_fill(const KParams& kp, const Tables<T>& tb, long long p0, long long p1,
+                              const int64_t* loff, int32_t* lobs, double* lrloc, hipStream_t s) {
+  long long blocks = (p1 - p0 + 255) / 256;
+  if (blocks < 1) return hipSuccess;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_search_fill<T>, dim3((unsigned)blocks), dim3(256), 0, s, kp, tb, p0, p1, loff,
+                     lobs, lrloc);
+  return hipGetLastError();
+}
+
+template hipError_t launch_search_count<float>(const KParams&, const Tables<float>&, int32_t*, hipStream_t);
+template hipError_t launch_search_count<double>(const KParams&, const Tables<double>&, int32_t*, hipStream_t);
+template hipError_t launch_search_fill<float>(const KParams&, const Tables<float>&, long long, long long,
+                                              const int64_t*, int32_t*, double*, hipStream_t);
+template hipError_t launch_search_fill<double>(const KParams&, const Tables<double>&, long long, long long,
+                                               const int64_t*, int32_t*, double*, hipStream_t);
+
+// ------------------------------------------------------------------ scans
+// Exclusive scans over points: loff (sum of nloc, int64) and actidx (count
+// of points with nloc > 0), plus the compacted list act_p of active points.
+// Three phases: per-tile totals, one-block scan of tile totals, tile rescans.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ void block_excl_scan2(long long& a, int& b, long long* sa, int* sb,
+                                                 long long& tot_a, int& tot_b) {
+  const int t = threadIdx.x;
+  sa[t] = a;
+  sb[t] = b;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    long long va = t >= off ? sa[t - off] : 0;
+    int vb = t >= off ? sb[t - off] : 0;
+    __syncthreads();
+    sa[t] += va;
+    sb[t] += vb;
+    __syncthreads();
+  }
+  tot_a = sa[kScanThreads - 1];
+  tot_b = sb[kScanThreads - 1];
+  a = sa[t] - a;
+  b = sb[t] - b;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(long long P, const int32_t* nloc,
+                                                             long long* tsum, int* tcnt) {
+  __shared__ long long sa[kScanThreads];
+  __shared__ int sb[kScanThreads];
+  const long long base = (long long)blockIdx.x * kScanTile;
+  long long a = 0;
+  int b = 0;
+  for (int q = 0; q < kScanItems; ++q) {
+    long long p = base + (long long)q * kScanThreads + threadIdx.x;
+    if (p < P) {
+      int n = nloc[p];
+      a += n;
+      b += n > 0;
+    }
+  }
+  long long ta;
+  int tb;
+  block_excl_scan2(a, b, sa, sb, ta, tb);
+  if (threadIdx.x == 0) {
+    tsum[blockIdx.x] = ta;
+    tcnt[blockIdx.x] = tb;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_totals(int ntiles, long long* tsum, int* tcnt) {
+  __shared__ long long sa[kScanThreads];
+  __shared__ int sb[kScanThreads];
+  long long carry_a = 0;
+  int carry_b = 0;
+  for (int base = 0; base < ntiles; base += kScanThreads) {
+    int q = base + threadIdx.x;
+    long long a = q < ntiles ? tsum[q] : 0;
+    int b = q < ntiles ? tcnt[q] : 0;
+    long long ta;
+    int tb;
+    block_excl_scan2(a, b, sa, sb, ta, tb);
+    if (q < ntiles) {
+      tsum[q] = a + carry_a;
+      tcnt[q] = b + carry_b;
+    }
+    carry_a += ta;
+    carry_b += tb;
+  }
+  if (threadIdx.x == 0) {
+    tsum[ntiles] = carry_a;
+    tcnt[ntiles] = carry_b;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(long long P, const int32_t* nloc,
+                                                             const long long* tsum, const int* tcnt,
+                                                             int64_t* loff, int32_t* actidx, int32_t* act_p) {
+  __shared__ long long sa[kScanThreads];
+  __shared__ int sb[kScanThreads];
+  const long long base = (long long)blockIdx.x * kScanTile;
+  // each thread owns kScanItems consecutive points
+  const long long mine = base + (long long)threadIdx.x * kScanItems;
+  int n[kScanItems];
+  long long a = 0;
+  int b = 0;
+  for (int q = 0; q < kScanItems; ++q) {
+    long long p = mine + q;
+    n[q] = p < P ? nloc[p] : 0;
+    a += n[q];
+    b += n[q] > 0;
+  }
+  long long ta;
+  int tb;
+  block_excl_scan2(a, b, sa, sb, ta, tb);
+  a += tsum[blockIdx.x];
+  b += tcnt[blockIdx.x];
+  for (int q = 0; q < kScanItems; ++q) {
+    long long p = mine + q;
+    if (p < P) {
+      loff[p] = a;
+      actidx[p] = b;
+      if (n[q] > 0) act_p[b] = (int32_t)p;
+      a += n[q];
+      b += n[q] > 0;
+    }
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    loff[P] = tsum[gridDim.x];
+    actidx[P] = tcnt[gridDim.x];
+  }
+}
+
+size_t scan_tmp_bytes(long long P) {
+  long long nt = (P + kScanTile - 1) / kScanTile + 1;
+  return (size_t)nt * (sizeof(long long) + sizeof(int)) + 64;
+}
+
+hipError_t launch_scan_points(long long P, const int32_t* nloc, int64_t* loff, int32_t* actidx,
+                              int32_t* act_p, void* tmp, size_t tmp_bytes, hipStream_t s) {
+  long long nt = (P + kScanTile - 1) / kScanTile;
+  if (nt < 1) nt = 1;
+  if (tmp_bytes < scan_tmp_bytes(P)) return hipErrorInvalidValue;
+  long long* tsum = (long long*)tmp;
+  int* tcnt = (int*)(tsum + nt + 1);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(kScanThreads), 0, s, P, nloc, tsum, tcnt);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, s, (int)nt, tsum, tcnt);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kScanThreads), 0, s, P, nloc, tsum, tcnt,
+                     loff, actidx, act_p);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------- obs transpose
+// obshdxf(nobs,K) column-major (member-major) -> yT[o*K + m] in FP64, so that
+// the K values of one obs are contiguous for the per-point gathers.
+template <typename T>
+__global__ void __launch_bounds__(256) k_transpose_obs(int nobs, int K, const T* hdxf, double* yT) {
+  __shared__ double tile[32][33];
+  const int o0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    int o = o0 + tx, m = m0 + r;
+    tile[r][tx] = (o < nobs && m < K) ? (double)hdxf[o + (long long)nobs * m] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    int o = o0 + r, m = m0 + tx;
+    if (o < nobs && m < K) yT[(long long)o * K + m] = tile[tx][r];
+  }
+}
+
+template <typename T>
+hipError_t launch_transpose_obs(int nobs, int K, const T* hdxf, double* yT, hipStream_t s) {
+  if (nobs <= 0) return hipSuccess;
+  dim3 grid((nobs + 31) / 32, (K + 31) / 32);
+  hipLaunchKernelGGL(k_transpose_obs<T>, grid, dim3(256), 0, s, nobs, K, hdxf, yT);
+  return hipGetLastError();
+}
+template hipError_t launch_transpose_obs<float>(int, int, const float*, double*, hipStream_t);
+template hipError_t launch_transpose_obs<double>(int, int, const double*, double*, hipStream_t);
+
+template <typename T>
+__global__ void k_copy(const T* src, T* dst, size_t n) {
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = src[q];
+}
+template <typename T>
+hipError_t launch_copy(const T* src, T* dst, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy<T>, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+template hipError_t launch_copy<float>(const float*, float*, size_t, hipStream_t);
+template hipError_t launch_copy<double>(const double*, double*, size_t, hipStream_t);
+
+}  // namespace letkf

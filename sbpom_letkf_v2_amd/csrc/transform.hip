@@ -1,0 +1,235 @@
+// transform.hip -- weights, ensemble transform and inflation of da_main
+// for a range of points (letkf/run/src/common_da.f90:74-113 and the
+// round_off of :506-533), given the eigen-decomposition C = V diag(lam) V^T
+// of each active point's ensemble-space matrix and b = Rlinv_dYf^T dep.
+//
+// With s_c = sqrt((K-1)/lam_c) the reference's
+//   Pa = V lam^-1 V^T, wvec = Pa b, Wmat = V diag(s) V^T      (common_letkf.f90:139-208)
+//   anal(j) = xf_mean + sum_i dxf(i) (wvec(i) + Wmat(i,j))    (common_da.f90:423-430)
+// are applied without forming Pa or Wmat:
+//   q = lam^-1 (V^T b), w = V q, z = diag(s) V^T dxf, anal = mean + dxf.w + V z
+// which costs 4K^2 per point instead of 4K^3.  Points without local obs take
+// letkf_core_noobs (W = sqrt(rho) I, w = 0; common_letkf.f90:37-57).
+//
+// Layout: fcst3d(nij1,nlev,K,nv3d) is read as [v][m][p] with p = i + nij1*k,
+// so a tile of consecutive points is one contiguous run per (m,v): each
+// workgroup stages TI consecutive points x all members x all variables in
+// LDS with coalesced loads, each wavefront then owns TI/4 points, lane j
+// owning members j and j+64.
+#include "letkf_internal.h"
+
+namespace letkf {
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Orders this wavefront's LDS writes before its later LDS reads; waves of
+// the workgroup work on different points, so no workgroup barrier here.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int kMaxVars = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, long long p0, long long p1, int TI,
+                                                   const int32_t* __restrict__ nloc,
+                                                   const int32_t* __restrict__ actidx, int a0,
+                                                   const double* __restrict__ lam, const double* __restrict__ V,
+                                                   const double* __restrict__ bvec, const T* __restrict__ X3,
+                                                   const T* __restrict__ X2, T* __restrict__ A3,
+                                                   T* __restrict__ A2) {
+  extern __shared__ double lds[];
+  const int K = kp.K, nv3 = kp.nv3d, nv2 = kp.nv2d, nvt = nv3 + nv2;
+  const long long P = kp.P;
+  const long long surf0 = (long long)kp.nij1 * (kp.nlev - 1);
+  const long long pt = p0 + (long long)blockIdx.x * TI;
+  const int ntile = (int)min((long long)TI, p1 - pt);
+  double* xs = lds;                                  // [nvt][K][TI]
+  double* scr = xs + (size_t)nvt * K * TI;           // per wave: q[K], z[nvt][K]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  double* q_s = scr + (size_t)wave * (1 + nvt) * K;
+  double* z_s = q_s + K;
+  // ---- stage the forecast tile ----
+  for (int e = t; e < nvt * K * TI; e += 256) {
+    const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
+    double x = 0.0;
+    if (tp < ntile) {
+      const long long p = pt + tp;
+      if (v < nv3)
+        x = (double)X3[p + P * (m + (long long)K * v)];
+      else if (p >= surf0)
+        x = (double)X2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))];
+    }
+    xs[e] = x;
+  }
+  __syncthreads();
+  const T rho_r = (T)kp.rho;
+  const double sqrt_rho = (double)sqrt(rho_r);
+  const double ap = (double)(T)kp.alpha_rtpp, as_ = (double)(T)kp.alpha_rtps;
+  const int m0 = lane, m1 = lane + 64;
+  const bool h0 = m0 < K, h1 = m1 < K;
+  const double rK = 1.0 / (double)K;
+  const int ppw = (TI + 3) / 4;
+  const double tmin = (double)(T)kp.tmin, tmax = (double)(T)kp.tmax;
+  const double smin = (double)(T)kp.smin, smax = (double)(T)kp.smax;
+  for (int jj = 0; jj < ppw; ++jj) {
+    const int tp = wave * ppw + jj;
+    if (tp >= ntile) break;
+    const long long p = pt + tp;
+    const int k = (int)(p / kp.nij1), i = (int)(p - (long long)k * kp.nij1);
+    const bool surf = p >= surf0;
+    const int nvp = surf ? nvt : nv3;
+    const int n = nloc[p];
+    // per-variable forecast mean and perturbations (members m0, m1)
+    double dx0[kMaxVars], dx1[kMaxVars], fm[kMaxVars], fs[kMaxVars], ta0[kMaxVars], ta1[kMaxVars];
+    _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+      const double x0 = h0 ? xs[((size_t)v * K + m0) * TI + tp] : 0.0;
+      const double x1 = h1 ? xs[((size_t)v * K + m1) * TI + tp] : 0.0;
+      fm[v] = wsum(x0 + x1) * rK;
+      dx0[v] = h0 ? x0 - fm[v] : 0.0;
+      dx1[v] = h1 ? x1 - fm[v] : 0.0;
+      fs[v] = sqrt(wsum(dx0[v] * dx0[v] + dx1[v] * dx1[v]) / (double)(K - 1));
+    }
+    if (n == 0) {
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+        ta0[v] = fm[v] + dx0[v] * sqrt_rho;
+        ta1[v] = fm[v] + dx1[v] * sqrt_rho;
+      }
+    } else {
+      const long long sl = (long long)(actidx[p] - a0);
+      const double* Vb = V + sl * K * K;
+      const double* lb = lam + sl * K;
+      const double* bb = bvec + sl * K;
+      // pass A over V by columns: q_c = (V^T b)_c / lam_c, z_c[v] = s_c (V^T dx[v])_c
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+        if (h0) z_s[v * K + m0] = dx0[v];
+        if (h1) z_s[v * K + m1] = dx1[v];
+      }
+      wave_sync();
+      double qa0 = 0.0, qa1 = 0.0, za0[kMaxVars], za1[kMaxVars];
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) za0[v] = za1[v] = 0.0;
+      for (int r = 0; r < K; ++r) {
+        const double v0 = h0 ? Vb[(long long)r * K + m0] : 0.0;
+        const double v1 = h1 ? Vb[(long long)r * K + m1] : 0.0;
+        const double br = bb[r];
+        qa0 = fma(v0, br, qa0);
+        qa1 = fma(v1, br, qa1);
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+          const double dr = z_s[v * K + r];
+          za0[v] = fma(v0, dr, za0[v]);
+          za1[v] = fma(v1, dr, za1[v]);
+        }
+      }
+      wave_sync();
+      const double km1 = (double)(T)(K - 1);
+      if (h0) {
+        const double l0 = lb[m0], s0 = sqrt(km1 / l0);
+        q_s[m0] = qa0 / l0;
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) z_s[v * K + m0] = s0 * za0[v];
+      }
+      if (h1) {
+        const double l1 = lb[m1], s1 = sqrt(km1 / l1);
+        q_s[m1] = qa1 / l1;
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) z_s[v * K + m1] = s1 * za1[v];
+      }
+      wave_sync();
+      // pass B over V by rows: w_j = (V q)_j, t_j[v] = (V z[v])_j
+      double w0 = 0.0, w1 = 0.0, tb0[kMaxVars], tb1[kMaxVars];
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) tb0[v] = tb1[v] = 0.0;
+      for (int c = 0; c < K; ++c) {
+        const double v0 = h0 ? Vb[(long long)m0 * K + c] : 0.0;
+        const double v1 = h1 ? Vb[(long long)m1 * K + c] : 0.0;
+        const double qc = q_s[c];
+        w0 = fma(v0, qc, w0);
+        w1 = fma(v1, qc, w1);
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+          const double zc = z_s[v * K + c];
+          tb0[v] = fma(v0, zc, tb0[v]);
+          tb1[v] = fma(v1, zc, tb1[v]);
+        }
+      }
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+        const double mu = wsum(dx0[v] * w0 + dx1[v] * w1);
+        ta0[v] = fm[v] + mu + tb0[v];
+        ta1[v] = fm[v] + mu + tb1[v];
+      }
+      wave_sync();
+    }
+    // analysis mean / spread / perturbation, RTPP, RTPS, round_off
+    const bool clampable = kp.roff && k >= 1 && !((T)tb.phi1[i] == (T)1);
+    _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+      const double am = wsum(h0 * ta0[v] + h1 * ta1[v]) * rK;
+      const double da0 = h0 ? ta0[v] - am : 0.0, da1 = h1 ? ta1[v] - am : 0.0;
+      const double asd = sqrt(wsum(da0 * da0 + da1 * da1) / (double)(K - 1));
+      double y0 = ta0[v], y1 = ta1[v];
+      if (ap > 0.0) {
+        y0 = am + ap * dx0[v] + (1.0 - ap) * da0;
+        y1 = am + ap * dx1[v] + (1.0 - ap) * da1;
+      }
+      if (as_ > 0.0) {
+        const double f = asd == 0.0 ? 0.0 : as_ * fs[v] / asd + (1.0 - as_);
+        y0 = am + da0 * f;
+        y1 = am + da1 * f;
+      }
+      if (clampable && v < nv3) {
+        if (v == kp.iv3d_t - 1) {
+          y0 = fmin(fmax(y0, tmin), tmax);
+          y1 = fmin(fmax(y1, tmin), tmax);
+        } else if (v == kp.iv3d_s - 1) {
+          y0 = fmin(fmax(y0, smin), smax);
+          y1 = fmin(fmax(y1, smin), smax);
+        }
+      }
+      if (h0) xs[((size_t)v * K + m0) * TI + tp] = y0;
+      if (h1) xs[((size_t)v * K + m1) * TI + tp] = y1;
+    }
+  }
+  __syncthreads();
+  // ---- write the analysis tile ----
+  for (int e = t; e < nvt * K * TI; e += 256) {
+    const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
+    if (tp >= ntile) continue;
+    const long long p = pt + tp;
+    if (v < nv3)
+      A3[p + P * (m + (long long)K * v)] = (T)xs[e];
+    else if (p >= surf0)
+      A2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))] = (T)xs[e];
+  }
+}
+
+template <typename T>
+hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0, long long p1,
+                            const int32_t* nloc, const int32_t* actidx, int a0, const double* lam,
+                            const double* V, const double* bvec, const T* fcst3d, const T* fcst2d, T* anal3d,
+                            T* anal2d, hipStream_t s) {
+  if (p1 <= p0) return hipSuccess;
+  const int nvt = kp.nv3d + kp.nv2d;
+  if (nvt > kMaxVars) return hipErrorInvalidValue;
+  const int TI = kp.K <= 64 ? 16 : 8;
+  const size_t lds = sizeof(double) * ((size_t)nvt * kp.K * TI + 4 * (size_t)(1 + nvt) * kp.K);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_transform<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  const long long blocks = (p1 - p0 + TI - 1) / TI;
+  hipLaunchKernelGGL(k_transform<T>, dim3((unsigned)blocks), dim3(256), lds, s, kp, tb, p0, p1, TI, nloc, actidx,
+                     a0, lam, V, bvec, fcst3d, fcst2d, anal3d, anal2d);
+  return hipGetLastError();
+}
+
+template hipError_t launch_transform<float>(const KParams&, const Tables<float>&, long long, long long,
+                                            const int32_t*, const int32_t*, int, const double*, const double*,
+                                            const double*, const float*, const float*, float*, float*,
+                                            hipStream_t);
+template hipError_t launch_transform<double>(const KParams&, const Tables<double>&, long long, long long,
+                                             const int32_t*, const int32_t*, int, const double*, const double*,
+                                             const double*, const double*, const double*, double*, double*,
+                                             hipStream_t);
+
+}  // namespace letkf

@@ -1,0 +1,148 @@
+"""Host-side mirror of the reference's analysis entry point.
+
+`da_main(problem)` plays the role of
+    CALL da_main(fcst3d, fcst2d, anal3d, anal2d)      (letkf/run/src/main.f90:92)
+with the module state of common_setting.f90 passed explicitly as a
+`synth.Problem`-like object.  Host numpy arrays go through the engine's
+host entry point (letkf_da_main), torch tensors on the GPU through the
+device entry point (letkf_da_main_device).  Errors raise EngineError with the
+engine's status code instead of STOP.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FORM_BATCHED, FORM_PLAIN, PREC_BF16X3, PREC_FP32, PREC_FP64, check
+
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+FORMS = {"plain": FORM_PLAIN, "batched": FORM_BATCHED}
+
+
+def _shape_of(a):
+    """Fortran shape of a numpy array or of a tensor holding a Fortran layout."""
+    if isinstance(a, np.ndarray):
+        return a.shape
+    return tuple(reversed(a.shape))
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def make_params(pr, precision="fp64", form="plain") -> _lib.Params:
+    p = pr.params
+    nij1, nlev = _shape_of(pr.depth1)
+    nlon, nlat = _shape_of(pr.nobsgrd)
+    f3 = _shape_of(pr.fcst3d)
+    f2 = _shape_of(pr.fcst2d)
+    prm = _lib.Params()
+    check(_lib.lib().letkf_params_default(C.byref(prm), int(p.get("real_kind", 8))))
+    prm.nmem = f3[2]
+    prm.nlon, prm.nlat, prm.nlev, prm.nij1 = nlon, nlat, nlev, nij1
+    prm.nv3d = f3[3]
+    prm.nv2d = f2[2] if len(f2) == 3 else 0
+    for k in ("iv3d_t", "iv3d_s", "id_z_obs", "roff"):
+        if k in p:
+            setattr(prm, k, int(p[k]))
+    for k in ("pi", "re", "cov_infl_mul", "alpha_rtpp", "alpha_rtps", "sigma_obs",
+              "sigma_obsv", "dist_zero", "dist_zerov", "tmin", "tmax", "smin", "smax"):
+        if k in p:
+            setattr(prm, k, float(p[k]))
+    prm.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    prm.form = FORMS[form] if isinstance(form, str) else int(form)
+    return prm
+
+
+class Engine:
+    """One engine handle (device workspaces + a HIP stream)."""
+
+    def __init__(self, params: _lib.Params):
+        self.params = params
+        h = C.c_void_p()
+        check(_lib.lib().letkf_create(C.byref(params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.lib().letkf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        check(_lib.lib().letkf_get_stats(self.h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+    # -------------------------------------------------------- host arrays
+    def da_main_host(self, pr):
+        dt = np.float32 if self.params.real_kind == 4 else np.float64
+        keep = []
+
+        def P(a, dtype=dt):
+            a = np.asfortranarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        g = _lib.Grid(*(P(getattr(pr, n)) for n in (
+            "lon1", "lat1", "ri1", "rj1", "dlon1", "dlat1", "phi1", "depth1")))
+        o = _lib.Obs(int(pr.obselm.shape[0]), P(pr.nobsgrd, np.int32), P(pr.obselm, np.int32),
+                     *(P(getattr(pr, n)) for n in ("obslon", "obslat", "obslev", "obserr",
+                                                    "obsdep", "obshdxf")))
+        f3 = np.asfortranarray(pr.fcst3d, dtype=dt)
+        f2 = np.asfortranarray(pr.fcst2d, dtype=dt)
+        a3 = np.zeros_like(f3, order="F")
+        a2 = np.zeros_like(f2, order="F")
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        check(_lib.lib().letkf_da_main(self.h, C.byref(g), C.byref(o), vp(f3), vp(f2), vp(a3), vp(a2)))
+        return a3, a2
+
+    # ------------------------------------------------------ device arrays
+    def da_main_device(self, pr, anal3d=None, anal2d=None, stream=None):
+        import torch
+        dt = torch.float32 if self.params.real_kind == 4 else torch.float64
+        for n in ("lon1", "fcst3d", "obshdxf"):
+            a = getattr(pr, n)
+            if not (_is_torch(a) and a.is_cuda):
+                raise TypeError(f"{n} must be a torch tensor on the GPU")
+            if a.dtype != dt:
+                raise TypeError(f"{n} has dtype {a.dtype}, real_kind needs {dt}")
+        ptr = lambda a: C.c_void_p(a.data_ptr()) if a is not None and a.numel() else None
+        g = _lib.Grid(*(ptr(getattr(pr, n).contiguous()) for n in (
+            "lon1", "lat1", "ri1", "rj1", "dlon1", "dlat1", "phi1", "depth1")))
+        o = _lib.Obs(int(pr.obselm.numel()), ptr(pr.nobsgrd), ptr(pr.obselm),
+                     *(ptr(getattr(pr, n)) for n in ("obslon", "obslat", "obslev", "obserr",
+                                                     "obsdep", "obshdxf")))
+        if anal3d is None:
+            anal3d = torch.empty_like(pr.fcst3d)
+        if anal2d is None:
+            anal2d = torch.empty_like(pr.fcst2d)
+        s = C.c_void_p(stream) if stream else None
+        check(_lib.lib().letkf_da_main_device(self.h, C.byref(g), C.byref(o), ptr(pr.fcst3d),
+                                              ptr(pr.fcst2d), ptr(anal3d), ptr(anal2d), s))
+        return anal3d, anal2d
+
+
+def da_main(pr, precision="fp64", form="plain"):
+    """Analysis of one problem; returns (anal3d, anal2d) in the same kind
+    of container as the inputs."""
+    eng = Engine(make_params(pr, precision, form))
+    try:
+        if _is_torch(pr.fcst3d):
+            return eng.da_main_device(pr)
+        return eng.da_main_host(pr)
+    finally:
+        eng.close()
