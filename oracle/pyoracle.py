@@ -105,6 +105,10 @@ def ref():
     if _ref is None:
         if not os.path.exists(REF_LIB):
             return None
+        # MKL's default threading layer clashes with the libgomp that torch
+        # loads into the same process (garbage results); the reference's
+        # BLAS calls run inside its own OpenMP loop, i.e. sequentially.
+        os.environ["MKL_THREADING_LAYER"] = "SEQUENTIAL"
         _ref = C.CDLL(REF_LIB)
         F = C.POINTER(C.c_float)
         D = C.POINTER(C.c_double)

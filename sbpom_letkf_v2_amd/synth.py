@@ -129,8 +129,10 @@ def _blur(x: torch.Tensor, dim: int, w: torch.Tensor) -> torch.Tensor:
     rad = (w.numel() - 1) // 2
     xt = x.movedim(dim, -1)
     shp = xt.shape
-    flat = xt.reshape(-1, 1, shp[-1])
-    flat = torch.nn.functional.pad(flat, (rad, rad), mode="circular")
+    n = shp[-1]
+    flat = xt.reshape(-1, 1, n)
+    wrap = torch.arange(-rad, n + rad, device=x.device) % n     # any radius, any n
+    flat = flat.index_select(2, wrap)
     out = torch.nn.functional.conv1d(flat, w.view(1, 1, -1).to(x.dtype))
     return out.reshape(shp).movedim(-1, dim)
 
