@@ -1,0 +1,219 @@
+"""Benchmark of the LETKF local analysis (da_main) on one MI355X.
+
+Workload (default): 512 x 512 x 40 grid, K = 64 members, U/V/T/S + SSH,
+2e5 observations (1.5e5 SST/SSH on 20-cell satellite-like stripes + 1250
+full-depth T/S profiles), RTPP 0.9, synthetic Gaussian-random-field
+ensemble generated on the GPU.  A step is one full analysis of every grid
+point, inputs resident in HBM.
+
+Forms:
+  plain    FP64 throughout, K x K eigensolve at every point with local obs
+  batched  the chosen form (see --form); until it exists this is plain
+
+Prints one JSON line (rank 0).  Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
+
+METRIC = "analysed grid pts/s at K=64: plain FP64 vs batched mixed-prec; max err vs FP64"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--form", default="plain")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU time of the oracle baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def run_form(prd, precision, form, steps, warmup):
+    import torch
+    from sbpom_letkf_v2_amd import engine
+    prm = engine.make_params(prd, precision, form)
+    prm.stage_timing = 1
+    eng = engine.Engine(prm)
+    a3 = torch.empty_like(prd.fcst3d)
+    a2 = torch.empty_like(prd.fcst2d)
+    for w in range(warmup):
+        t = time.perf_counter()
+        eng.da_main_device(prd, a3, a2)
+        log(f"  warmup {w}: {time.perf_counter() - t:.2f}s {eng.stats()}")
+    torch.cuda.synchronize()
+    tot = {k: 0.0 for k in ("ms_search", "ms_gram", "ms_eig", "ms_transform", "ms_total")}
+    t0 = time.perf_counter()
+    for s in range(steps):
+        eng.da_main_device(prd, a3, a2)
+        st = eng.stats()
+        for k in tot:
+            tot[k] += st[k]
+        log(f"  step {s}: {st['ms_total']:.1f} ms")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    return a3, a2, wall / steps, {k: v / steps for k, v in tot.items()}, st
+
+
+def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2):
+    """FP64 oracle (the C restatement, OpenMP) on a uniform random sample of
+    points of the same workload; also returns the GPU-vs-oracle error at the
+    sampled points (full-size parity)."""
+    import numpy as np
+    import torch
+    import pyoracle as po
+    nij1, nlev = pr_host_tables.depth1.shape
+    P = nij1 * nlev
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    rng = np.random.default_rng(7)
+
+    def sample(npt):
+        pts = np.sort(rng.choice(P, npt, replace=False))
+        pk = (pts // nij1).astype(np.int32)
+        pi = (pts % nij1).astype(np.int32)
+        tp = torch.as_tensor(pts, device=prd.fcst3d.device)
+        # fcst3d tensor memory is [v][m][p]
+        x3 = prd.fcst3d.reshape(nv3, K, P)[:, :, tp].permute(2, 0, 1).contiguous().cpu().numpy()
+        ti = torch.as_tensor(pi.astype(np.int64), device=prd.fcst3d.device)
+        x2 = prd.fcst2d.reshape(nv2, K, nij1)[:, :, ti].permute(2, 0, 1).contiguous().cpu().numpy() \
+            if nv2 else np.zeros((npt, 0, K))
+        g3 = a3_gpu.reshape(nv3, K, P)[:, :, tp].permute(2, 0, 1).contiguous().cpu().numpy()
+        g2 = a2_gpu.reshape(nv2, K, nij1)[:, :, ti].permute(2, 0, 1).contiguous().cpu().numpy() \
+            if nv2 else None
+        return pi, pk, x3, x2, g3, g2
+
+    # calibrate on a small sample, then size the timed sample to ~target_s
+    pi, pk, x3, x2, _, _ = sample(64 * cores)
+    t0 = time.perf_counter()
+    po.analyse_points(pr_host_tables, pi, pk, x3, x2, nthreads=cores)
+    dt = time.perf_counter() - t0
+    npt = int(min(P, max(64 * cores, 64 * cores * target_s / max(dt, 1e-3))))
+    pi, pk, x3, x2, g3, g2 = sample(npt)
+    t0 = time.perf_counter()
+    rc, xa3, xa2, nl = po.analyse_points(pr_host_tables, pi, pk, x3, x2, nthreads=cores)
+    dt = time.perf_counter() - t0
+    assert rc == 0, rc
+    err3 = float(np.abs(g3 - xa3).max() / np.abs(xa3).max())
+    surf = pk == nlev - 1
+    err2 = float(np.abs(g2[surf] - xa2[surf]).max() / np.abs(xa2[surf]).max()) if (nv2 and surf.any()) else 0.0
+    return dict(value=npt / dt, unit="grid points/s", cores=cores, kind="port",
+                sample=f"{npt} uniformly sampled (i,k) points of the same workload, "
+                       f"FP64 C oracle with {cores} OpenMP threads, {dt:.1f} s"), max(err3, err2), npt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the analysis shards "
+              "by grid points across GPUs with no exchange inside the path -- run N=1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import pyoracle as po
+    from sbpom_letkf_v2_amd import synth
+
+    t0 = time.perf_counter()
+    log(f"generating workload {args.workload} on the GPU")
+    prd = synth.config(args.workload, device="cuda", real_kind=8)
+    torch.cuda.synchronize()
+    meta = prd.meta
+    K, nv3, nv2 = meta["nmem"], meta["nv3d"], meta["nv2d"]
+    nij1 = meta["nlon"] * meta["nlat"]
+    P = nij1 * meta["nlev"]
+    log(f"generated in {time.perf_counter() - t0:.1f}s: P={P} K={K} nobs={meta['nobs']}")
+
+    log(f"timing chosen form precision={args.precision} form={args.form}: "
+        f"{args.warmup} warmup + {args.steps} steps")
+    a3, a2, sec, stage, st = run_form(prd, args.precision, args.form, args.steps, args.warmup)
+    log(f"  {sec * 1e3:.1f} ms/step, stages {stage}")
+    result_extra = {}
+    if not args.no_plain and (args.precision, args.form) != ("fp64", "plain"):
+        log("timing plain FP64 form")
+        p3, p2, psec, pstage, pst = run_form(prd, "fp64", "plain", max(1, args.steps // 2), 1)
+        d3 = (a3 - p3).abs()
+        result_extra["plain_fp64"] = {"value": P / psec, "ms_per_step": psec * 1e3,
+                                      "stage_ms": pstage}
+        result_extra["speedup_vs_plain_fp64"] = psec / sec
+        del p3, p2
+    # --- roofline of the dominant kernel (stage times measured with HIP events) ---
+    L = st["local_obs"]
+    na = st["active_points"]
+    nch = -(-P // max(1024, min((1 << 31) // (8 * K * K), 65536)))
+    stages = {
+        "gram": dict(ms=stage["ms_gram"], work=2.0 * L * K * K, unit="TFLOP/s", bound="mfma",
+                     peak=PEAK_FP64_TFLOPS, kernel="k_gram_fp64"),
+        "eig": dict(ms=stage["ms_eig"], work=9.0 * na * K ** 3, unit="TFLOP/s", bound="mfma",
+                    peak=PEAK_FP64_TFLOPS, kernel="k_eig_fp64"),
+        "transform": dict(ms=stage["ms_transform"], work=2.0 * 8 * K * (P * nv3 + nij1 * nv2),
+                          unit="GB/s", bound="hbm", peak=PEAK_HBM_GBS, kernel="k_transform"),
+    }
+    dom_name = max(stages, key=lambda k: stages[k]["ms"])
+    dom = stages[dom_name]
+    scale = 1e12 if dom["unit"] == "TFLOP/s" else 1e9
+    achieved = dom["work"] / (dom["ms"] * 1e-3) / scale
+    per_launch_ms = dom["ms"] / nch
+    roofline = {"bound": dom["bound"], "achieved": achieved, "peak": dom["peak"], "unit": dom["unit"],
+                "frac": achieved / dom["peak"], "traffic": None, "kernel": dom["kernel"],
+                "launches_per_step": nch, "avg_launch_ms": per_launch_ms,
+                "work_per_launch": dom["work"] / nch}
+    stage_roofline = {}
+    for k, v in stages.items():
+        sc = 1e12 if v["unit"] == "TFLOP/s" else 1e9
+        a = v["work"] / max(v["ms"] * 1e-3, 1e-12) / sc
+        stage_roofline[k] = {"ms": v["ms"], "achieved": a, "unit": v["unit"], "frac": a / v["peak"]}
+    # --- CPU baseline + full-size parity at the sampled points ---
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        log("CPU baseline (FP64 oracle) on a bounded sample")
+        pn_tabs = synth.Problem(
+            params=dict(prd.params), **{k: getattr(prd, k) for k in synth.Problem.ARRAYS
+                                        if k not in ("fcst3d", "fcst2d")},
+            fcst3d=torch.zeros((nv3, K, 1, 1), dtype=torch.float64),
+            fcst2d=torch.zeros((max(nv2, 1), K, 1), dtype=torch.float64)).numpy()
+        opr = po.Problem(params=dict(pn_tabs.params),
+                         **{k: getattr(pn_tabs, k) for k in synth.Problem.ARRAYS})
+        ref_a3 = a3 if (args.precision, args.form) == ("fp64", "plain") else None
+        if ref_a3 is not None:
+            cpu, parity, npt = cpu_baseline(opr, prd, a3, a2, args.cpu_seconds, K, nv3, nv2)
+            log(f"  cpu {cpu['value']:.0f} pts/s on {cpu['cores']} cores; GPU vs oracle max rel err "
+                f"{parity:.2e} over {npt} points")
+    out = {
+        "metric": METRIC, "value": P / sec, "unit": "grid points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else f"{args.precision}+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {meta['nlon']}x{meta['nlat']}x{meta['nlev']} "
+                               f"K={K} nv3d={nv3} nv2d={nv2} nobs={meta['nobs']} "
+                               f"({meta['obs_layout']}), {args.precision}/{args.form}",
+                   "points": P, "active_points": na, "local_obs": L,
+                   "precision": args.precision, "form": args.form},
+        "roofline": roofline, "stage_roofline": stage_roofline, "cpu_baseline": cpu,
+        "parity_max_rel_err_vs_fp64_oracle_sample": parity,
+    }
+    out.update(result_extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

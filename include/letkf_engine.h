@@ -70,6 +70,8 @@ typedef struct letkf_params {
   double sigma_obs, sigma_obsv; /* [m]                       (setting:94-95) */
   double dist_zero, dist_zerov; /* cut-offs, r_size values   (setting:97-98) */
   double tmin, tmax, smin, smax;/* round_off bounds          (setting:130-131) */
+  int32_t stage_timing;         /* 1: time each stage with HIP events into */
+                                /* letkf_stats_t (adds event records only)  */
 } letkf_params_t;
 
 /* Local grid of the analysis (common_setting.f90:168-176). */

@@ -73,6 +73,16 @@ struct letkf_handle {
   // staging for the host-pointer entry point
   DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
   letkf_stats_t stats{};
+  std::vector<hipEvent_t> events;  // stage-timing marks, grown on demand
+  hipError_t ensure_events(size_t n) {
+    while (events.size() < n) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      events.push_back(e);
+    }
+    return hipSuccess;
+  }
 };
 
 static int check_params(const letkf_params_t* p) {
@@ -156,6 +166,7 @@ extern "C" int letkf_destroy(letkf_handle_t* h) {
                    &h->lrloc, &h->G,    &h->lam,    &h->V,     &h->bvec,     &h->status, &h->bounds,
                    &h->h_grid, &h->h_obs, &h->h_f3,  &h->h_f2,  &h->h_a3,     &h->h_a2};
   for (DevBuf* b : all) b->release();
+  for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return LETKF_OK;
@@ -263,12 +274,18 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
   HIP_TRY(hipMemsetAsync(h->status.ptr, 0, sizeof(int) * 4, s));
 
+  const bool timing = p.stage_timing != 0;
+  // stage marks: [0,1] search count+scan, then per chunk 5 marks
+  auto mark = [&](size_t q) -> hipError_t { return timing ? hipEventRecord(h->events[q], s) : hipSuccess; };
   // obs-major FP64 copy of obshdxf
   HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
   // count_nobsl for every point, then the scans
+  if (timing) HIP_TRY(h->ensure_events(2));
+  HIP_TRY(mark(0));
   HIP_TRY(launch_search_count<T>(kp, tb, h->nloc.as<int32_t>(), s));
   HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
                              h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
+  HIP_TRY(mark(1));
 
   // chunks of consecutive points sized so that the per-point K x K
   // workspaces (G, V) stay within ~2 GiB each
@@ -301,22 +318,33 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
 
+  if (timing) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
   for (long long c = 0; c < nch; ++c) {
+    const size_t e0 = 2 + 5 * (size_t)c;
     const long long p0 = bpts[c], p1 = bpts[c + 1];
     const long long lbase = bv[2 * c];
     const int a0 = (int)bv[2 * c + 1];
     const int na = (int)(bv[2 * (c + 1) + 1] - a0);
+    HIP_TRY(mark(e0));
     if (na > 0) {
       HIP_TRY(launch_search_fill<T>(kp, tb, p0, p1, h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
                                     h->lrloc.as<double>(), s));
+    }
+    HIP_TRY(mark(e0 + 1));
+    if (na > 0) {
       HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
                                   lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na, h->G.as<double>(),
                                   h->bvec.as<double>(), s));
+    }
+    HIP_TRY(mark(e0 + 2));
+    if (na > 0) {
       HIP_TRY(launch_eig_fp64(K, na, h->G.as<double>(), nullptr, h->lam.as<double>(), h->V.as<double>(),
                               h->status.as<int>(), s));
     }
+    HIP_TRY(mark(e0 + 3));
     HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,
                                 h->lam.as<double>(), h->V.as<double>(), h->bvec.as<double>(), f3, f2, a3, a2, s));
+    HIP_TRY(mark(e0 + 4));
   }
   HIP_TRY(hipEventRecord(ev1, s));
   int status = 0;
@@ -327,6 +355,19 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   st.ms_total = ms;
+  if (timing) {
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, h->events[0], h->events[1]));
+    st.ms_search = t;
+    for (long long c = 0; c < nch; ++c) {
+      const size_t e0 = 2 + 5 * (size_t)c;
+      double* dst[4] = {&st.ms_search, &st.ms_gram, &st.ms_eig, &st.ms_transform};
+      for (int q = 0; q < 4; ++q) {
+        HIP_TRY(hipEventElapsedTime(&t, h->events[e0 + q], h->events[e0 + q + 1]));
+        *dst[q] += t;
+      }
+    }
+  }
   st.gram_flops = 2.0 * (double)st.local_obs * K * K;
   st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
   h->stats = st;
