@@ -11,13 +11,13 @@
 #include <algorithm>
 #include <vector>
 
-#include "letkf_internal.h"
+#include "letkf_handle.h"
 
 using namespace letkf;
 
 static thread_local char g_err[512] = "";
 
-static int set_err(int code, const char* fmt, ...) {
+int set_err(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof g_err, fmt, ap);
@@ -25,65 +25,13 @@ static int set_err(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t _e = (expr);                                                                    \
-    if (_e != hipSuccess)                                                                      \
-      return set_err(LETKF_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),     \
-                     __FILE__, __LINE__);                                                      \
-  } while (0)
-
-namespace {
-
-// Grow-only device buffer.
-struct DevBuf {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-  hipError_t ensure(size_t want) {
-    if (want <= bytes) return hipSuccess;
-    if (ptr) {
-      hipError_t e = hipFree(ptr);
-      if (e != hipSuccess) return e;
-      ptr = nullptr;
-      bytes = 0;
-    }
-    want = want < 256 ? 256 : want;
-    hipError_t e = hipMalloc(&ptr, want);
-    if (e == hipSuccess) bytes = want;
-    return e;
-  }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    bytes = 0;
-  }
-  template <typename X>
-  X* as() const {
-    return (X*)ptr;
-  }
-};
-
-}  // namespace
-
-struct letkf_handle {
-  letkf_params_t prm;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  DevBuf nloc, loff, actidx, act_p, scan_tmp, yT, lobs, lrloc, G, lam, V, bvec, status, bounds;
-  // staging for the host-pointer entry point
-  DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
-  letkf_stats_t stats{};
-  std::vector<hipEvent_t> events;  // stage-timing marks, grown on demand
-  hipError_t ensure_events(size_t n) {
-    while (events.size() < n) {
-      hipEvent_t e;
-      hipError_t r = hipEventCreate(&e);
-      if (r != hipSuccess) return r;
-      events.push_back(e);
-    }
-    return hipSuccess;
-  }
-};
+int status_to_rc(int status) {
+  if (status & ST_NONFINITE) return set_err(LETKF_ERR_NONFINITE, "non-finite ensemble-space matrix");
+  if (status & ST_EIG_NOCONV) return set_err(LETKF_ERR_EIG, "eigensolve did not converge");
+  if (status & ST_EIG_NEG) return set_err(LETKF_ERR_EIG, "all eigenvalues negative");
+  if (status & ST_CAPACITY) return set_err(LETKF_ERR_CAPACITY, "local problem beyond engine capacity");
+  return LETKF_OK;
+}
 
 static int check_params(const letkf_params_t* p) {
   if (!p) return set_err(LETKF_ERR_ARG, "params is NULL");
@@ -162,11 +110,7 @@ extern "C" int letkf_destroy(letkf_handle_t* h) {
   if (!h) return LETKF_OK;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  DevBuf* all[] = {&h->nloc, &h->loff,  &h->actidx, &h->act_p, &h->scan_tmp, &h->yT,   &h->lobs,
-                   &h->lrloc, &h->G,    &h->lam,    &h->V,     &h->bvec,     &h->status, &h->bounds,
-                   &h->h_grid, &h->h_obs, &h->h_f3,  &h->h_f2,  &h->h_a3,     &h->h_a2};
-  for (DevBuf* b : all) b->release();
-  for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
+  h->release_all();
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return LETKF_OK;
@@ -371,10 +315,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   st.gram_flops = 2.0 * (double)st.local_obs * K * K;
   st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
   h->stats = st;
-  if (status & ST_NONFINITE) return set_err(LETKF_ERR_NONFINITE, "non-finite ensemble-space matrix");
-  if (status & ST_EIG_NOCONV) return set_err(LETKF_ERR_EIG, "eigensolve did not converge");
-  if (status & ST_EIG_NEG) return set_err(LETKF_ERR_EIG, "all eigenvalues negative");
-  return LETKF_OK;
+  return status_to_rc(status);
 }
 
 static int check_call(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,

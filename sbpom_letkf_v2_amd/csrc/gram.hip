@@ -4,32 +4,73 @@
 //   G = Rlinv_dYf^T hdxf + (K-1)/rho I              (:112-126)
 //   b = Rlinv_dYf^T dep                              (the right-hand side of
 //       wvec = Pa (Rlinv_dYf^T dep), an exact re-association of :162-184)
-// One workgroup per point; the local obs rows are gathered from the
-// obs-major FP64 copy of obshdxf through LDS in panels of 2048/KT obs.
+// One workgroup per local problem; its rows are staged through LDS in panels
+// of 2048/KT rows.
 #include "letkf_internal.h"
 
 namespace letkf {
 
 
+// Where the rows of one local problem come from.
+//  ObsSource: the da_main path -- row l of point p is obs lobs[o0+l] of the
+//    sorted tables (obs-major FP64 copy yT), rdiag = REAL(obserr**2, r_size),
+//    rloc from the search.
+//  CsrSource: letkf_core_batched -- rows given directly.
+template <typename T>
+struct ObsSource {
+  Tables<T> tb;
+  const double* yT;
+  const int32_t* act_p;
+  const int64_t* loff;
+  long long lbase;
+  const int32_t* lobs;
+  const double* lrloc;
+  int a0;
+  long long o0;
+  __device__ int begin(int b) {
+    const int p = act_p[a0 + b];
+    o0 = loff[p] - lbase;
+    return (int)(loff[p + 1] - loff[p]);
+  }
+  __device__ void row(int l, int K, int m, double& y, double& rl, double& rd) const {
+    const int io = lobs[o0 + l];
+    rl = lrloc[o0 + l];
+    const T er = tb.oerr[io];
+    rd = (double)(T)(er * er);
+    y = yT[(long long)io * K + m];
+  }
+  __device__ double dep(int l) const { return (double)tb.odep[lobs[o0 + l]]; }
+};
+
+template <typename T>
+struct CsrSource {
+  const int64_t* offs;
+  const T *hdxf, *rdiag, *rloc, *depv;
+  long long o0;
+  __device__ int begin(int b) {
+    o0 = offs[b];
+    return (int)(offs[b + 1] - offs[b]);
+  }
+  __device__ void row(int l, int K, int m, double& y, double& rl, double& rd) const {
+    const long long q = o0 + l;
+    y = (double)hdxf[q * K + m];
+    rl = (double)rloc[q];
+    rd = (double)rdiag[q];
+  }
+  __device__ double dep(int l) const { return (double)depv[o0 + l]; }
+};
+
 // KT: padded ensemble tile (64 or 128); each of the 16x16 threads owns a
-// TT x TT block of G with TT = KT/16.
-template <typename T, int KT>
-__global__ void __launch_bounds__(256) k_gram_fp64(KParams kp, Tables<T> tb, const double* __restrict__ yT,
-                                                   const int32_t* __restrict__ act_p,
-                                                   const int64_t* __restrict__ loff, long long lbase,
-                                                   const int32_t* __restrict__ lobs,
-                                                   const double* __restrict__ lrloc, int a0,
-                                                   double* __restrict__ G, double* __restrict__ bvec) {
+// TT x TT block of G with TT = KT/16.  diag = (K-1)/rho in r_size.
+template <int KT, typename Src>
+__global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, double* __restrict__ G,
+                                                   double* __restrict__ bvec, int32_t* __restrict__ nrow) {
   constexpr int TT = KT / 16;
   constexpr int kPanel = 2048 / KT;  // 32 obs at KT=64, 16 at KT=128 (33 KiB LDS)
   __shared__ double As[kPanel][KT + 1];  // Rlinv_dYf rows
   __shared__ double Bs[kPanel][KT + 1];  // hdxf rows
   __shared__ double ds[kPanel];
-  const int K = kp.K;
-  const int a = a0 + blockIdx.x;
-  const int p = act_p[a];
-  const long long o0 = loff[p] - lbase;
-  const int n = (int)(loff[p + 1] - loff[p]);
+  const int n = src.begin(blockIdx.x);
   const int t = threadIdx.x;
   const int tr = t >> 4, tc = t & 15;
   double acc[TT][TT];
@@ -40,22 +81,18 @@ __global__ void __launch_bounds__(256) k_gram_fp64(KParams kp, Tables<T> tb, con
   double bacc = 0.0;
   for (int l0 = 0; l0 < n; l0 += kPanel) {
     const int nb = min(kPanel, n - l0);
-    // stage panel: thread t loads member m = t % KT of obs rows t / KT, ...
     for (int q = t; q < kPanel * KT; q += 256) {
       const int l = q / KT, m = q - l * KT;
       double av = 0.0, bv = 0.0;
       if (l < nb && m < K) {
-        const int io = lobs[o0 + l0 + l];
-        const double rl = lrloc[o0 + l0 + l];
-        const T er = tb.oerr[io];
-        const double rd = (double)(T)(er * er);  // rdiag = REAL(obserr**2, r_size)
-        bv = yT[(long long)io * K + m];
-        av = bv * rl / rd;
+        double rl, rd;
+        src.row(l0 + l, K, m, bv, rl, rd);
+        av = bv * rl / rd;  // Rlinv_dYf = hdxf * rloc / rdiag (common_letkf.f90:95)
       }
       As[l][m] = av;
       Bs[l][m] = bv;
     }
-    if (t < kPanel) ds[t] = t < nb ? (double)tb.odep[lobs[o0 + l0 + t]] : 0.0;
+    if (t < kPanel) ds[t] = t < nb ? src.dep(l0 + t) : 0.0;
     __syncthreads();
     for (int l = 0; l < nb; ++l) {
       double ar[TT], bc[TT];
@@ -72,7 +109,6 @@ __global__ void __launch_bounds__(256) k_gram_fp64(KParams kp, Tables<T> tb, con
       for (int l = 0; l < nb; ++l) bacc = fma(As[l][t], ds[l], bacc);
     __syncthreads();
   }
-  const double diag = (double)((T)(K - 1) / (T)kp.rho);
   double* Gp = G + (long long)blockIdx.x * K * K;
 #pragma unroll
   for (int x = 0; x < TT; ++x) {
@@ -85,21 +121,41 @@ __global__ void __launch_bounds__(256) k_gram_fp64(KParams kp, Tables<T> tb, con
     }
   }
   if (t < K) bvec[(long long)blockIdx.x * K + t] = bacc;
+  if (t == 0 && nrow) nrow[blockIdx.x] = n;
+}
+
+template <typename Src>
+static hipError_t launch_gram_src(int K, double diag, const Src& src, int na, double* G, double* bvec,
+                                  int32_t* nrow, hipStream_t s) {
+  if (na <= 0) return hipSuccess;
+  if (K <= 64)
+    hipLaunchKernelGGL((k_gram_fp64<64, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow);
+  else
+    hipLaunchKernelGGL((k_gram_fp64<128, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow);
+  return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb, const double* yT, const int32_t* act_p,
                             const int64_t* loff, long long lbase, const int32_t* lobs, const double* lrloc,
                             int a0, int na, double* G, double* bvec, hipStream_t s) {
-  if (na <= 0) return hipSuccess;
-  if (kp.K <= 64)
-    hipLaunchKernelGGL((k_gram_fp64<T, 64>), dim3(na), dim3(256), 0, s, kp, tb, yT, act_p, loff, lbase,
-                       lobs, lrloc, a0, G, bvec);
-  else
-    hipLaunchKernelGGL((k_gram_fp64<T, 128>), dim3(na), dim3(256), 0, s, kp, tb, yT, act_p, loff, lbase,
-                       lobs, lrloc, a0, G, bvec);
-  return hipGetLastError();
+  ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
+  const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
+  return launch_gram_src(kp.K, diag, src, na, G, bvec, nullptr, s);
 }
+
+template <typename T>
+hipError_t launch_gram_csr(int K, double rho, int nb, const int64_t* offs, const T* hdxf, const T* rdiag,
+                           const T* rloc, const T* dep, double* G, double* bvec, int32_t* nrow, hipStream_t s) {
+  CsrSource<T> src{offs, hdxf, rdiag, rloc, dep, 0};
+  const double diag = (double)((T)(K - 1) / (T)rho);
+  return launch_gram_src(K, diag, src, nb, G, bvec, nrow, s);
+}
+template hipError_t launch_gram_csr<float>(int, double, int, const int64_t*, const float*, const float*,
+                                           const float*, const float*, double*, double*, int32_t*, hipStream_t);
+template hipError_t launch_gram_csr<double>(int, double, int, const int64_t*, const double*, const double*,
+                                            const double*, const double*, double*, double*, int32_t*,
+                                            hipStream_t);
 
 template hipError_t launch_gram_fp64<float>(const KParams&, const Tables<float>&, const double*,
                                             const int32_t*, const int64_t*, long long, const int32_t*,

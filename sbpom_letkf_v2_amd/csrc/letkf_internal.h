@@ -70,6 +70,32 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
                             const int32_t* lobs, const double* lrloc, int a0,
                             int na, double* G, double* bvec, hipStream_t s);
 
+// Same products for letkf_core_batched: problem b owns rows offs[b]..offs[b+1)
+// of hdxf (K values per row); nrow[b] receives the row count.
+template <typename T>
+hipError_t launch_gram_csr(int K, double rho, int nb, const int64_t* offs,
+                           const T* hdxf, const T* rdiag, const T* rloc,
+                           const T* dep, double* G, double* bvec,
+                           int32_t* nrow, hipStream_t s);
+
+// From the eigen-decomposition: wvec = V lam^-1 V^T b and
+// Wmat = V diag(sqrt((K-1)/lam)) V^T, or letkf_core_noobs where nrow == 0.
+template <typename T>
+hipError_t launch_form_weights(int K, double rho, int nb, const int32_t* nrow,
+                               const double* lam, const double* V,
+                               const double* bvec, T* wvec, T* wmat,
+                               hipStream_t s);
+
+// AOEI and obs binning pre-passes.
+template <typename T>
+hipError_t launch_aoei(int nobs, T* err, const T* dep, const T* sprd,
+                       hipStream_t s);
+template <typename T>
+hipError_t launch_obs_cells(int nlon, int nlat, const T* lon, const T* lat,
+                            int nobs, const T* olon, const T* olat,
+                            int32_t* idx, int32_t* idy, int32_t* key,
+                            int32_t* counts, hipStream_t s);
+
 // Batched symmetric eigensolve: G [na][K][K] (upper triangle read) ->
 // lam [na][K] descending, V [na][K][K] (V[r][j] = component r of vector j).
 hipError_t launch_eig_fp64(int K, int na, const double* G, double* work,
