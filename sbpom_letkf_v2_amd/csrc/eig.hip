@@ -202,16 +202,25 @@ __device__ bool eig_wave(int n, double* Z, int ld, double* d, double* e) {
   return ok;
 }
 
-// One matrix per 64-lane workgroup.  Dynamic LDS: n*(n+1) + 2n doubles.
-__global__ void __launch_bounds__(64) k_eig_fp64(int n, const double* __restrict__ G, double* __restrict__ lam,
-                                                 double* __restrict__ V, int* __restrict__ status) {
+// One matrix per 64-lane workgroup.  Slot b starts at G + b*K*K; its size
+// is K, or in the dual form the point's local-obs count when below K.
+// Dynamic LDS: K*(K+1) + 2K doubles.
+__global__ void __launch_bounds__(64) k_eig_fp64(int K, const double* __restrict__ G, double* __restrict__ lam,
+                                                 double* __restrict__ V, int* __restrict__ status, int dual,
+                                                 const int32_t* __restrict__ nloc,
+                                                 const int32_t* __restrict__ act_p, int a0) {
   extern __shared__ double lds[];
+  int n = K;
+  if (dual) {
+    const int nl = nloc[act_p[a0 + blockIdx.x]];
+    n = nl < K ? nl : K;
+  }
   const int ld = n + 1;
   double* Z = lds;
   double* d = Z + n * ld;
   double* e = d + n;
   const int lane = threadIdx.x;
-  const double* Gb = G + (long long)blockIdx.x * n * n;
+  const double* Gb = G + (long long)blockIdx.x * K * K;
   // symmetric copy from the upper triangle (DSYEV('V','U') semantics)
   for (int q = lane; q < n * n; q += 64) {
     const int r = q / n, c = q - r * n;
@@ -230,8 +239,8 @@ __global__ void __launch_bounds__(64) k_eig_fp64(int n, const double* __restrict
     ok = eig_wave(n, Z, ld, d, e);
   }
   // descending order; ties keep the reverse of DSYEV's ascending order
-  double* lb = lam + (long long)blockIdx.x * n;
-  double* Vb = V + (long long)blockIdx.x * n * n;
+  double* lb = lam + (long long)blockIdx.x * K;
+  double* Vb = V + (long long)blockIdx.x * K * K;
   double dmax = -INFINITY;
   bool finite = true;
   for (int j = 0; j < n; ++j) {
@@ -255,9 +264,8 @@ __global__ void __launch_bounds__(64) k_eig_fp64(int n, const double* __restrict
   }
 }
 
-hipError_t launch_eig_fp64(int K, int na, const double* G, double* work, double* lam, double* V, int* status,
-                           hipStream_t s) {
-  (void)work;
+hipError_t launch_eig_fp64(int K, int na, const double* G, double* lam, double* V, int* status, int dual,
+                           const int32_t* nloc, const int32_t* act_p, int a0, hipStream_t s) {
   if (na <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * ((size_t)K * (K + 1) + 2 * (size_t)K);
   if (lds > 65536) {
@@ -265,7 +273,7 @@ hipError_t launch_eig_fp64(int K, int na, const double* G, double* work, double*
                                        (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_eig_fp64, dim3(na), dim3(64), lds, s, K, G, lam, V, status);
+  hipLaunchKernelGGL(k_eig_fp64, dim3(na), dim3(64), lds, s, K, G, lam, V, status, dual, nloc, act_p, a0);
   return hipGetLastError();
 }
 

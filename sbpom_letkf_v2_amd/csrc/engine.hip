@@ -219,6 +219,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   HIP_TRY(hipMemsetAsync(h->status.ptr, 0, sizeof(int) * 4, s));
 
   const bool timing = p.stage_timing != 0;
+  const int dual = p.form == LETKF_FORM_BATCHED;
   // stage marks: [0,1] search count+scan, then per chunk 5 marks
   auto mark = [&](size_t q) -> hipError_t { return timing ? hipEventRecord(h->events[q], s) : hipSuccess; };
   // obs-major FP64 copy of obshdxf
@@ -278,16 +279,18 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     if (na > 0) {
       HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
                                   lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na, h->G.as<double>(),
-                                  h->bvec.as<double>(), s));
+                                  h->bvec.as<double>(), dual, s));
     }
     HIP_TRY(mark(e0 + 2));
     if (na > 0) {
-      HIP_TRY(launch_eig_fp64(K, na, h->G.as<double>(), nullptr, h->lam.as<double>(), h->V.as<double>(),
-                              h->status.as<int>(), s));
+      HIP_TRY(launch_eig_fp64(K, na, h->G.as<double>(), h->lam.as<double>(), h->V.as<double>(),
+                              h->status.as<int>(), dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0, s));
     }
     HIP_TRY(mark(e0 + 3));
     HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,
-                                h->lam.as<double>(), h->V.as<double>(), h->bvec.as<double>(), f3, f2, a3, a2, s));
+                                h->lam.as<double>(), h->V.as<double>(), h->bvec.as<double>(), f3, f2, a3, a2, dual,
+                                h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
+                                h->yT.as<double>(), s));
     HIP_TRY(mark(e0 + 4));
   }
   HIP_TRY(hipEventRecord(ev1, s));

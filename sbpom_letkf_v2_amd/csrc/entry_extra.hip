@@ -108,8 +108,8 @@ extern "C" int letkf_core_batched(letkf_handle_t* h, int32_t nb, const int64_t* 
     HIP_TRY(launch_gram_csr<float>(K, rho, nb, offs, (const float*)hdxf, (const float*)rdiag, (const float*)rloc,
                                    (const float*)dep, h->G.as<double>(), h->bvec.as<double>(),
                                    h->nrow.as<int32_t>(), s));
-  HIP_TRY(launch_eig_fp64(K, nb, h->G.as<double>(), nullptr, h->lam.as<double>(), h->V.as<double>(),
-                          h->status.as<int>(), s));
+  HIP_TRY(launch_eig_fp64(K, nb, h->G.as<double>(), h->lam.as<double>(), h->V.as<double>(), h->status.as<int>(),
+                          0, nullptr, nullptr, 0, s));
   if (h->prm.real_kind == 8)
     HIP_TRY(launch_form_weights<double>(K, rho, nb, h->nrow.as<int32_t>(), h->lam.as<double>(), h->V.as<double>(),
                                         h->bvec.as<double>(), (double*)wvec, (double*)wmat, s));

@@ -40,6 +40,13 @@ struct ObsSource {
     y = yT[(long long)io * K + m];
   }
   __device__ double dep(int l) const { return (double)tb.odep[lobs[o0 + l]]; }
+  __device__ void row_of(int io, int K, int m, double& y) const { y = yT[(long long)io * K + m]; }
+  // row l of B = diag(sqrt(rloc/rdiag)) Y: obs index and sqrt weight
+  __device__ void brow(int l, int& io, double& sw) const {
+    io = lobs[o0 + l];
+    const T er = tb.oerr[io];
+    sw = sqrt(lrloc[o0 + l] / (double)(T)(er * er));
+  }
 };
 
 template <typename T>
@@ -58,18 +65,31 @@ struct CsrSource {
     rd = (double)rdiag[q];
   }
   __device__ double dep(int l) const { return (double)depv[o0 + l]; }
+  __device__ void brow(int l, int& io, double& sw) const {
+    io = (int)(o0 + l);
+    sw = sqrt((double)rloc[o0 + l] / (double)rdiag[o0 + l]);
+  }
+  __device__ void row_of(int io, int K, int m, double& y) const { y = (double)hdxf[(long long)io * K + m]; }
 };
 
 // KT: padded ensemble tile (64 or 128); each of the 16x16 threads owns a
 // TT x TT block of G with TT = KT/16.  diag = (K-1)/rho in r_size.
+//
+// dual != 0 (batched form): a point with n < K local obs instead gets the
+// n x n matrix M = B B^T, B = diag(sqrt(rloc/rdiag)) Y, summed over the K
+// members, and bvec = diag(sqrt(rloc/rdiag)) dep; both are stored with
+// row pitch n.  C = (K-1)/rho I + B^T B shares its nonzero spectrum with M.
 template <int KT, typename Src>
 __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, double* __restrict__ G,
-                                                   double* __restrict__ bvec, int32_t* __restrict__ nrow) {
+                                                   double* __restrict__ bvec, int32_t* __restrict__ nrow,
+                                                   int dual) {
   constexpr int TT = KT / 16;
   constexpr int kPanel = 2048 / KT;  // 32 obs at KT=64, 16 at KT=128 (33 KiB LDS)
   __shared__ double As[kPanel][KT + 1];  // Rlinv_dYf rows
   __shared__ double Bs[kPanel][KT + 1];  // hdxf rows
   __shared__ double ds[kPanel];
+  __shared__ double sws[KT];
+  __shared__ int ios[KT];
   const int n = src.begin(blockIdx.x);
   const int t = threadIdx.x;
   const int tr = t >> 4, tc = t & 15;
@@ -79,6 +99,57 @@ __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, 
 #pragma unroll
     for (int y = 0; y < TT; ++y) acc[x][y] = 0.0;
   double bacc = 0.0;
+  if (dual && n < K) {
+    // ---- M = B B^T over members, panels of kPanel members ----
+    for (int l = t; l < n; l += 256) {
+      int io;
+      double sw;
+      src.brow(l, io, sw);
+      ios[l] = io;
+      sws[l] = sw;
+      bvec[(long long)blockIdx.x * K + l] = sw * src.dep(l);
+    }
+    __syncthreads();
+    for (int m0 = 0; m0 < K; m0 += kPanel) {
+      const int pm = min(kPanel, K - m0);
+      for (int q = t; q < kPanel * KT; q += 256) {
+        const int l = q / kPanel, mm = q - l * kPanel;
+        double v = 0.0;
+        if (l < n && mm < pm) {
+          double y;
+          src.row_of(ios[l], K, m0 + mm, y);
+          v = sws[l] * y;
+        }
+        Bs[mm][l] = v;
+      }
+      __syncthreads();
+      for (int mm = 0; mm < pm; ++mm) {
+        double ar[TT], bc[TT];
+#pragma unroll
+        for (int x = 0; x < TT; ++x) ar[x] = Bs[mm][tr * TT + x];
+#pragma unroll
+        for (int y = 0; y < TT; ++y) bc[y] = Bs[mm][tc * TT + y];
+#pragma unroll
+        for (int x = 0; x < TT; ++x)
+#pragma unroll
+          for (int y = 0; y < TT; ++y) acc[x][y] = fma(ar[x], bc[y], acc[x][y]);
+      }
+      __syncthreads();
+    }
+    double* Mp = G + (long long)blockIdx.x * K * K;
+#pragma unroll
+    for (int x = 0; x < TT; ++x) {
+      const int r = tr * TT + x;
+      if (r >= n) continue;
+#pragma unroll
+      for (int y = 0; y < TT; ++y) {
+        const int c = tc * TT + y;
+        if (c < n) Mp[r * n + c] = acc[x][y];
+      }
+    }
+    if (t == 0 && nrow) nrow[blockIdx.x] = n;
+    return;
+  }
   for (int l0 = 0; l0 < n; l0 += kPanel) {
     const int nb = min(kPanel, n - l0);
     for (int q = t; q < kPanel * KT; q += 256) {
@@ -126,22 +197,22 @@ __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, 
 
 template <typename Src>
 static hipError_t launch_gram_src(int K, double diag, const Src& src, int na, double* G, double* bvec,
-                                  int32_t* nrow, hipStream_t s) {
+                                  int32_t* nrow, int dual, hipStream_t s) {
   if (na <= 0) return hipSuccess;
   if (K <= 64)
-    hipLaunchKernelGGL((k_gram_fp64<64, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow);
+    hipLaunchKernelGGL((k_gram_fp64<64, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow, dual);
   else
-    hipLaunchKernelGGL((k_gram_fp64<128, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow);
+    hipLaunchKernelGGL((k_gram_fp64<128, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow, dual);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb, const double* yT, const int32_t* act_p,
                             const int64_t* loff, long long lbase, const int32_t* lobs, const double* lrloc,
-                            int a0, int na, double* G, double* bvec, hipStream_t s) {
+                            int a0, int na, double* G, double* bvec, int dual, hipStream_t s) {
   ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
-  return launch_gram_src(kp.K, diag, src, na, G, bvec, nullptr, s);
+  return launch_gram_src(kp.K, diag, src, na, G, bvec, nullptr, dual, s);
 }
 
 template <typename T>
@@ -149,7 +220,7 @@ hipError_t launch_gram_csr(int K, double rho, int nb, const int64_t* offs, const
                            const T* rloc, const T* dep, double* G, double* bvec, int32_t* nrow, hipStream_t s) {
   CsrSource<T> src{offs, hdxf, rdiag, rloc, dep, 0};
   const double diag = (double)((T)(K - 1) / (T)rho);
-  return launch_gram_src(K, diag, src, nb, G, bvec, nrow, s);
+  return launch_gram_src(K, diag, src, nb, G, bvec, nrow, 0, s);
 }
 template hipError_t launch_gram_csr<float>(int, double, int, const int64_t*, const float*, const float*,
                                            const float*, const float*, double*, double*, int32_t*, hipStream_t);
@@ -159,9 +230,9 @@ template hipError_t launch_gram_csr<double>(int, double, int, const int64_t*, co
 
 template hipError_t launch_gram_fp64<float>(const KParams&, const Tables<float>&, const double*,
                                             const int32_t*, const int64_t*, long long, const int32_t*,
-                                            const double*, int, int, double*, double*, hipStream_t);
+                                            const double*, int, int, double*, double*, int, hipStream_t);
 template hipError_t launch_gram_fp64<double>(const KParams&, const Tables<double>&, const double*,
                                              const int32_t*, const int64_t*, long long, const int32_t*,
-                                             const double*, int, int, double*, double*, hipStream_t);
+                                             const double*, int, int, double*, double*, int, hipStream_t);
 
 }  // namespace letkf

@@ -68,7 +68,8 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
                             const double* yT, const int32_t* act_p,
                             const int64_t* loff, long long lbase,
                             const int32_t* lobs, const double* lrloc, int a0,
-                            int na, double* G, double* bvec, hipStream_t s);
+                            int na, double* G, double* bvec, int dual,
+                            hipStream_t s);
 
 // Same products for letkf_core_batched: problem b owns rows offs[b]..offs[b+1)
 // of hdxf (K values per row); nrow[b] receives the row count.
@@ -96,10 +97,14 @@ hipError_t launch_obs_cells(int nlon, int nlat, const T* lon, const T* lat,
                             int32_t* idx, int32_t* idy, int32_t* key,
                             int32_t* counts, hipStream_t s);
 
-// Batched symmetric eigensolve: G [na][K][K] (upper triangle read) ->
-// lam [na][K] descending, V [na][K][K] (V[r][j] = component r of vector j).
-hipError_t launch_eig_fp64(int K, int na, const double* G, double* work,
-                           double* lam, double* V, int* status, hipStream_t s);
+// Batched symmetric eigensolve: slot b holds an n_b x n_b matrix at
+// G + b*K*K with row pitch n_b (upper triangle read) -> lam[b*K + j]
+// descending, V + b*K*K row pitch n_b (V[r][j] = component r of vector j).
+// n_b = K, or (dual != 0) n_b = min(nloc[act_p[a0+b]], K).
+hipError_t launch_eig_fp64(int K, int na, const double* G, double* lam,
+                           double* V, int* status, int dual,
+                           const int32_t* nloc, const int32_t* act_p, int a0,
+                           hipStream_t s);
 
 // Weights + ensemble transform + RTPP/RTPS + round_off for points [p0,p1).
 template <typename T>
@@ -108,7 +113,10 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb,
                             const int32_t* actidx, int a0, const double* lam,
                             const double* V, const double* bvec,
                             const T* fcst3d, const T* fcst2d, T* anal3d,
-                            T* anal2d, hipStream_t s);
+                            T* anal2d, int dual, const int64_t* loff,
+                            long long lbase, const int32_t* lobs,
+                            const double* lrloc, const double* yT,
+                            hipStream_t s);
 
 template <typename T>
 hipError_t launch_copy(const T* src, T* dst, size_t n, hipStream_t s);

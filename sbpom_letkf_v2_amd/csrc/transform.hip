@@ -35,6 +35,19 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int kMaxVars = 8;
 
+// What the dual branch needs to rebuild B = diag(sqrt(rloc/rdiag)) Y of a
+// point: its local list (lobs/lrloc from loff[p]-lbase) and the obs-major
+// FP64 copy of obshdxf.  enabled == 0 for the plain form.
+struct DualRows {
+  int enabled;
+  const int64_t* loff;
+  long long lbase;
+  const int32_t* lobs;
+  const double* lrloc;
+  const double* yT;
+  double alpha;  // (K-1)/rho in r_size
+};
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, long long p0, long long p1, int TI,
                                                    const int32_t* __restrict__ nloc,
@@ -42,7 +55,7 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
                                                    const double* __restrict__ lam, const double* __restrict__ V,
                                                    const double* __restrict__ bvec, const T* __restrict__ X3,
                                                    const T* __restrict__ X2, T* __restrict__ A3,
-                                                   T* __restrict__ A2) {
+                                                   T* __restrict__ A2, DualRows dr) {
   extern __shared__ double lds[];
   const int K = kp.K, nv3 = kp.nv3d, nv2 = kp.nv2d, nvt = nv3 + nv2;
   const long long P = kp.P;
@@ -50,10 +63,12 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
   const long long pt = p0 + (long long)blockIdx.x * TI;
   const int ntile = (int)min((long long)TI, p1 - pt);
   double* xs = lds;                                  // [nvt][K][TI]
-  double* scr = xs + (size_t)nvt * K * TI;           // per wave: q[K], z[nvt][K]
+  double* scr = xs + (size_t)nvt * K * TI;           // per wave: 2 banks of (1+nvt)*K + K ints
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  double* q_s = scr + (size_t)wave * (1 + nvt) * K;
-  double* z_s = q_s + K;
+  double* q_s = scr + (size_t)wave * (2 * (1 + nvt) * K + K / 2 + 1);
+  double* z_s = q_s + K;                             // bank A: q[K], z[nvt][K]
+  double* y_s = q_s + (size_t)(1 + nvt) * K;         // bank B: [nvt+1][K]
+  int* io_s = (int*)(y_s + (size_t)(1 + nvt) * K);   // K obs indices
   // ---- stage the forecast tile ----
   for (int e = t; e < nvt * K * TI; e += 256) {
     const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
@@ -100,6 +115,131 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
         ta0[v] = fm[v] + dx0[v] * sqrt_rho;
         ta1[v] = fm[v] + dx1[v] * sqrt_rho;
       }
+    } else if (dr.enabled && n < K) {
+      // ---- dual form: n x n problem M = B B^T = U diag(s) U^T ----
+      //   w      = B^T U diag(1/(alpha+s)) U^T dt,   dt = sqrt(rloc/rdiag) dep
+      //   W dx   = sqrt(rho) dx + sqrt(K-1) B^T U diag(g(s)) U^T B dx
+      //   g(s)   = ((alpha+s)^-1/2 - alpha^-1/2)/s, evaluated without cancellation
+      const long long sl = (long long)(actidx[p] - a0);
+      const double* Ub = V + sl * K * K;   // pitch n
+      const double* sb = lam + sl * K;
+      const double* db = bvec + sl * K;
+      const long long o0 = dr.loff[p] - dr.lbase;
+      const bool g0 = lane < n, g1 = lane + 64 < n;
+      int io0 = 0, io1 = 0;
+      double sw0 = 0.0, sw1 = 0.0;
+      if (g0) {
+        io0 = dr.lobs[o0 + lane];
+        const T er = tb.oerr[io0];
+        sw0 = sqrt(dr.lrloc[o0 + lane] / (double)(T)(er * er));
+        io_s[lane] = io0;
+      }
+      if (g1) {
+        io1 = dr.lobs[o0 + lane + 64];
+        const T er = tb.oerr[io1];
+        sw1 = sqrt(dr.lrloc[o0 + lane + 64] / (double)(T)(er * er));
+        io_s[lane + 64] = io1;
+      }
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+        if (h0) z_s[v * K + m0] = dx0[v];
+        if (h1) z_s[v * K + m1] = dx1[v];
+      }
+      wave_sync();
+      // (1) lanes over obs: t[v]_l = sw_l (y_l . dx[v])
+      double t0[kMaxVars], t1[kMaxVars];
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) t0[v] = t1[v] = 0.0;
+      for (int m = 0; m < K; ++m) {
+        const double y0 = g0 ? dr.yT[(long long)io0 * K + m] : 0.0;
+        const double y1 = g1 ? dr.yT[(long long)io1 * K + m] : 0.0;
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+          const double dm = z_s[v * K + m];
+          t0[v] = fma(y0, dm, t0[v]);
+          t1[v] = fma(y1, dm, t1[v]);
+        }
+      }
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+        if (g0) y_s[v * K + lane] = sw0 * t0[v];
+        if (g1) y_s[v * K + lane + 64] = sw1 * t1[v];
+      }
+      if (g0) y_s[nvp * K + lane] = db[lane];
+      if (g1) y_s[nvp * K + lane + 64] = db[lane + 64];
+      wave_sync();
+      // (2) lanes over eigenpairs c: u[v]_c = g(s_c) (U^T t[v])_c, q_c = (U^T dt)_c / (alpha + s_c)
+      double u0[kMaxVars], u1[kMaxVars], qd0 = 0.0, qd1 = 0.0;
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) u0[v] = u1[v] = 0.0;
+      for (int l = 0; l < n; ++l) {
+        const double U0 = g0 ? Ub[(long long)l * n + lane] : 0.0;
+        const double U1 = g1 ? Ub[(long long)l * n + lane + 64] : 0.0;
+        const double dl = y_s[nvp * K + l];
+        qd0 = fma(U0, dl, qd0);
+        qd1 = fma(U1, dl, qd1);
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+          const double tl = y_s[v * K + l];
+          u0[v] = fma(U0, tl, u0[v]);
+          u1[v] = fma(U1, tl, u1[v]);
+        }
+      }
+      wave_sync();
+      const double al = dr.alpha, sal = sqrt(al);
+      if (g0) {
+        const double sc = sb[lane], ap_ = al + sc, sap = sqrt(ap_);
+        const double gc = -1.0 / (sal * sap * (sal + sap));
+        q_s[lane] = qd0 / ap_;
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) z_s[v * K + lane] = gc * u0[v];
+      }
+      if (g1) {
+        const double sc = sb[lane + 64], ap_ = al + sc, sap = sqrt(ap_);
+        const double gc = -1.0 / (sal * sap * (sal + sap));
+        q_s[lane + 64] = qd1 / ap_;
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) z_s[v * K + lane + 64] = gc * u1[v];
+      }
+      wave_sync();
+      // (3) lanes over obs: vv[v]_l = sw_l (U u[v])_l, zq_l = sw_l (U q)_l
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) t0[v] = t1[v] = 0.0;
+      double zq0 = 0.0, zq1 = 0.0;
+      for (int c = 0; c < n; ++c) {
+        const double U0 = g0 ? Ub[(long long)lane * n + c] : 0.0;
+        const double U1 = g1 ? Ub[(long long)(lane + 64) * n + c] : 0.0;
+        const double qc = q_s[c];
+        zq0 = fma(U0, qc, zq0);
+        zq1 = fma(U1, qc, zq1);
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+          const double uc = z_s[v * K + c];
+          t0[v] = fma(U0, uc, t0[v]);
+          t1[v] = fma(U1, uc, t1[v]);
+        }
+      }
+      wave_sync();
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+        if (g0) y_s[v * K + lane] = sw0 * t0[v];
+        if (g1) y_s[v * K + lane + 64] = sw1 * t1[v];
+      }
+      if (g0) y_s[nvp * K + lane] = sw0 * zq0;
+      if (g1) y_s[nvp * K + lane + 64] = sw1 * zq1;
+      wave_sync();
+      // (4) lanes over members: r[v]_m = (Y^T vv[v])_m, w_m = (Y^T zq)_m
+      double r0[kMaxVars], r1[kMaxVars], w0 = 0.0, w1 = 0.0;
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) r0[v] = r1[v] = 0.0;
+      for (int l = 0; l < n; ++l) {
+        const long long row = (long long)io_s[l] * K;
+        const double y0 = h0 ? dr.yT[row + m0] : 0.0;
+        const double y1 = h1 ? dr.yT[row + m1] : 0.0;
+        const double zl = y_s[nvp * K + l];
+        w0 = fma(y0, zl, w0);
+        w1 = fma(y1, zl, w1);
+        _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+          const double vl = y_s[v * K + l];
+          r0[v] = fma(y0, vl, r0[v]);
+          r1[v] = fma(y1, vl, r1[v]);
+        }
+      }
+      const double skm1 = sqrt((double)(T)(K - 1)), srho = sqrt((double)(T)(K - 1) / al);
+      _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
+        const double mu = wsum(dx0[v] * w0 + dx1[v] * w1);
+        ta0[v] = fm[v] + mu + (srho * dx0[v] + skm1 * r0[v]);
+        ta1[v] = fm[v] + mu + (srho * dx1[v] + skm1 * r1[v]);
+      }
+      wave_sync();
     } else {
       const long long sl = (long long)(actidx[p] - a0);
       const double* Vb = V + sl * K * K;
@@ -206,12 +346,15 @@ template <typename T>
 hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0, long long p1,
                             const int32_t* nloc, const int32_t* actidx, int a0, const double* lam,
                             const double* V, const double* bvec, const T* fcst3d, const T* fcst2d, T* anal3d,
-                            T* anal2d, hipStream_t s) {
+                            T* anal2d, int dual, const int64_t* loff, long long lbase, const int32_t* lobs,
+                            const double* lrloc, const double* yT, hipStream_t s) {
   if (p1 <= p0) return hipSuccess;
   const int nvt = kp.nv3d + kp.nv2d;
   if (nvt > kMaxVars) return hipErrorInvalidValue;
   const int TI = kp.K <= 64 ? 16 : 8;
-  const size_t lds = sizeof(double) * ((size_t)nvt * kp.K * TI + 4 * (size_t)(1 + nvt) * kp.K);
+  const size_t lds = sizeof(double) * ((size_t)nvt * kp.K * TI +
+                                       4 * (2 * (size_t)(1 + nvt) * kp.K + kp.K / 2 + 1));
+  DualRows dr{dual, loff, lbase, lobs, lrloc, yT, (double)((T)(kp.K - 1) / (T)kp.rho)};
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_transform<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -219,17 +362,19 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0
   }
   const long long blocks = (p1 - p0 + TI - 1) / TI;
   hipLaunchKernelGGL(k_transform<T>, dim3((unsigned)blocks), dim3(256), lds, s, kp, tb, p0, p1, TI, nloc, actidx,
-                     a0, lam, V, bvec, fcst3d, fcst2d, anal3d, anal2d);
+                     a0, lam, V, bvec, fcst3d, fcst2d, anal3d, anal2d, dr);
   return hipGetLastError();
 }
 
 template hipError_t launch_transform<float>(const KParams&, const Tables<float>&, long long, long long,
                                             const int32_t*, const int32_t*, int, const double*, const double*,
-                                            const double*, const float*, const float*, float*, float*,
-                                            hipStream_t);
+                                            const double*, const float*, const float*, float*, float*, int,
+                                            const int64_t*, long long, const int32_t*, const double*,
+                                            const double*, hipStream_t);
 template hipError_t launch_transform<double>(const KParams&, const Tables<double>&, long long, long long,
                                              const int32_t*, const int32_t*, int, const double*, const double*,
                                              const double*, const double*, const double*, double*, double*,
-                                             hipStream_t);
+                                             int, const int64_t*, long long, const int32_t*, const double*,
+                                             const double*, hipStream_t);
 
 }  // namespace letkf

@@ -97,3 +97,119 @@ def test_nobs_zero_copies_forecast():
     prd = pr.to("cuda")
     a3, a2 = engine.da_main(prd)
     assert torch.equal(a3, prd.fcst3d) and torch.equal(a2, prd.fcst2d)
+
+
+@pytest.mark.parametrize("K,nobs,seed", [(20, 300, 51), (64, 700, 52), (100, 2500, 53)])
+def test_batched_form_matches_fp64_oracle(K, nobs, seed):
+    """Batched form: points with nobsl < K solve the nobsl x nobsl problem.
+    Same LETKF, so the FP64 tolerance is unchanged (1e-10)."""
+    _say(f"case batched K={K} nobs={nobs} seed={seed}")
+    pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=K, nobs=nobs, seed=seed,
+                            alpha_rtpp=0.7, alpha_rtps=0.3, land_frac=0.1, real_kind=8)
+    a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched")
+    act = nl[nl > 0]
+    _say(f"  n<K points {(act < K).sum()} n>=K points {(act >= K).sum()}")
+    assert (act < K).sum() > 0 and st["local_obs"] == int(nl.sum())
+    e3 = per_var_err(a3, a3o, axis_var=3)
+    e2 = per_var_err(a2, a2o, axis_var=2)
+    _say(f"  err3={e3} err2={e2}")
+    assert max(e3) <= 1e-10 and max(e2) <= 1e-10
+
+
+def _csr_problems(rng, K, sizes, dtype):
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    L = int(offs[-1])
+    Y = rng.standard_normal((max(L, 1), K))
+    for b in range(len(sizes)):
+        seg = slice(offs[b], offs[b + 1])
+        Y[seg] -= Y[seg].mean(1, keepdims=True)
+    rd = rng.uniform(0.3, 2.0, max(L, 1)) ** 2
+    rl = rng.uniform(0.01, 1.0, max(L, 1))
+    dp = rng.standard_normal(max(L, 1))
+    return offs, [np.asarray(a, dtype) for a in (Y, rd, rl, dp)]
+
+
+@pytest.mark.parametrize("K", [20, 64, 128])
+def test_core_batched_vs_oracle(K):
+    """letkf_core_batched (wvec, Wmat of common_letkf.f90:37-210) vs the
+    FP64 oracle's letkf_core per problem, including empty problems."""
+    import ctypes as C
+    from sbpom_letkf_v2_amd import _lib
+    _say(f"case core_batched K={K}")
+    rng = np.random.default_rng(K)
+    sizes = [0, 1, 3, K - 1, K, 2 * K + 5, 17, 0, 250]
+    offs, (Y, rd, rl, dp) = _csr_problems(rng, K, sizes, np.float64)
+    pr = synth.make_problem(nlon=8, nlat=6, nlev=2, nmem=K, nobs=10, seed=1)
+    prm = engine.make_params(pr)
+    prm.cov_infl_mul = 1.15
+    eng = engine.Engine(prm)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    tO, tY, trd, trl, tdp = dev(offs), dev(Y), dev(rd), dev(rl), dev(dp)
+    nb = len(sizes)
+    w = torch.empty((nb, K), dtype=torch.float64, device="cuda")
+    W = torch.empty((nb, K, K), dtype=torch.float64, device="cuda")
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    _lib.check(_lib.lib().letkf_core_batched(eng.h, nb, vp(tO), vp(tY), vp(trd), vp(trl), vp(tdp),
+                                             vp(w), vp(W)))
+    w, W = w.cpu().numpy(), W.cpu().numpy()
+    eng.close()
+    for b, n in enumerate(sizes):
+        seg = slice(offs[b], offs[b + 1])
+        if n == 0:
+            assert np.allclose(W[b], np.sqrt(1.15) * np.eye(K), atol=1e-15) and not w[b].any()
+            continue
+        rc, wo, Wo = po.letkf_core(Y[seg], rd[seg], rl[seg], dp[seg], 1.15)
+        assert np.abs(w[b] - wo).max() <= 1e-10 * max(np.abs(wo).max(), 1e-30)
+        assert np.abs(W[b] - Wo.T).max() <= 1e-10 * np.abs(Wo).max()   # wmat[b][i][j] = Wmat(i,j)
+
+
+def test_bin_obs_bit_exact_vs_oracle():
+    import ctypes as C
+    from sbpom_letkf_v2_amd import _lib
+    _say("case bin_obs")
+    rng = np.random.default_rng(77)
+    nlon, nlat = 37, 23
+    lon = 130 + np.arange(nlon) / 16
+    lat = 10 + np.arange(nlat) / 16
+    L2 = np.asfortranarray(np.repeat(lon[:, None], nlat, 1))
+    A2 = np.asfortranarray(np.repeat(lat[None, :], nlon, 0))
+    n = 5000
+    olon = rng.uniform(lon[0] - 0.05, lon[-1] + 0.05, n)
+    olat = rng.uniform(lat[0] - 0.05, lat[-1] + 0.05, n)
+    olon[:100] = lon[rng.integers(0, nlon, 100)]
+    idx_o, idy_o, perm_o, grd_o = po.bin_obs(L2, A2, olon, olat)
+    pr = synth.make_problem(nlon=8, nlat=6, nlev=2, nmem=4, nobs=10, seed=1)
+    eng = engine.Engine(engine.make_params(pr))
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    tL, tA = dev(L2.T), dev(A2.T)            # memory = Fortran (nlon,nlat)
+    to, ta = dev(olon), dev(olat)
+    idx = torch.empty(n, dtype=torch.int32, device="cuda")
+    idy = torch.empty_like(idx)
+    perm = torch.empty_like(idx)
+    grd = torch.empty((nlat, nlon), dtype=torch.int32, device="cuda")
+    ns = C.c_int32()
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    _lib.check(_lib.lib().letkf_bin_obs(eng.h, nlon, nlat, vp(tL), vp(tA), n, vp(to), vp(ta), vp(idx),
+                                        vp(idy), vp(perm), vp(grd), C.byref(ns)))
+    eng.close()
+    assert ns.value == len(perm_o)
+    assert np.array_equal(idx.cpu().numpy(), idx_o) and np.array_equal(idy.cpu().numpy(), idy_o)
+    assert np.array_equal(perm.cpu().numpy()[: ns.value], perm_o)
+    assert np.array_equal(grd.cpu().numpy().T, grd_o)
+
+
+def test_aoei_gpu_vs_oracle():
+    import ctypes as C
+    from sbpom_letkf_v2_amd import _lib
+    _say("case aoei")
+    rng = np.random.default_rng(3)
+    e = rng.uniform(0.1, 1, 1000)
+    d = rng.standard_normal(1000)
+    s = rng.uniform(0, 1, 1000)
+    pr = synth.make_problem(nlon=8, nlat=6, nlev=2, nmem=4, nobs=10, seed=1)
+    eng = engine.Engine(engine.make_params(pr))
+    te, td, ts = (torch.as_tensor(a).cuda() for a in (e, d, s))
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    _lib.check(_lib.lib().letkf_aoei(eng.h, 1000, vp(te), vp(td), vp(ts)))
+    eng.close()
+    assert np.array_equal(te.cpu().numpy(), po.aoei(e, d, s))
