@@ -160,7 +160,7 @@ def test_core_batched_vs_oracle(K):
             continue
         rc, wo, Wo = po.letkf_core(Y[seg], rd[seg], rl[seg], dp[seg], 1.15)
         assert np.abs(w[b] - wo).max() <= 1e-10 * max(np.abs(wo).max(), 1e-30)
-        assert np.abs(W[b] - Wo.T).max() <= 1e-10 * np.abs(Wo).max()   # wmat[b][i][j] = Wmat(i,j)
+        assert np.abs(W[b] - Wo).max() <= 1e-10 * np.abs(Wo).max()   # wmat[b][i][j] = Wmat(i,j)
 
 
 def test_bin_obs_bit_exact_vs_oracle():
