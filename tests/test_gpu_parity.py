@@ -99,13 +99,16 @@ def test_nobs_zero_copies_forecast():
     assert torch.equal(a3, prd.fcst3d) and torch.equal(a2, prd.fcst2d)
 
 
-@pytest.mark.parametrize("K,nobs,seed", [(20, 300, 51), (64, 700, 52), (100, 2500, 53)])
+@pytest.mark.parametrize("K,nobs,seed", [(20, 300, 51), (20, 100, 54), (64, 700, 52),
+                                         (100, 1800, 53), (128, 2000, 55)])
 def test_batched_form_matches_fp64_oracle(K, nobs, seed):
     """Batched form: points with nobsl < K solve the nobsl x nobsl problem.
-    Same LETKF, so the FP64 tolerance is unchanged (1e-10)."""
+    Same LETKF, so the FP64 tolerance is unchanged (1e-10).  A 1-cell /
+    1-level localisation scale gives a mix of nobsl < K and nobsl >= K."""
     _say(f"case batched K={K} nobs={nobs} seed={seed}")
     pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=K, nobs=nobs, seed=seed,
-                            alpha_rtpp=0.7, alpha_rtps=0.3, land_frac=0.1, real_kind=8)
+                            alpha_rtpp=0.7, alpha_rtps=0.3, land_frac=0.1, real_kind=8,
+                            sigma_h_cells=1.0, sigma_v=1.0)
     a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched")
     act = nl[nl > 0]
     _say(f"  n<K points {(act < K).sum()} n>=K points {(act >= K).sum()}")
