@@ -133,7 +133,7 @@ def main():
 
     t0 = time.perf_counter()
     log(f"generating workload {args.workload} on the GPU")
-    prd = synth.config(args.workload, device="cuda", real_kind=8)
+    prd = synth.config(args.workload, device="cuda", real_kind=8, progress=log)
     torch.cuda.synchronize()
     meta = prd.meta
     K, nv3, nv2 = meta["nmem"], meta["nv3d"], meta["nv2d"]

@@ -125,23 +125,22 @@ def _kernel(std: float, device) -> torch.Tensor:
 
 
 def _blur(x: torch.Tensor, dim: int, w: torch.Tensor) -> torch.Tensor:
-    """Circular convolution of x along dim with the odd kernel w."""
+    """Circular convolution of x along dim with the odd symmetric kernel w
+    (shift-and-add: a handful of elementwise passes, no convolution library)."""
     rad = (w.numel() - 1) // 2
-    xt = x.movedim(dim, -1)
-    shp = xt.shape
-    n = shp[-1]
-    flat = xt.reshape(-1, 1, n)
-    wrap = torch.arange(-rad, n + rad, device=x.device) % n     # any radius, any n
-    flat = flat.index_select(2, wrap)
-    out = torch.nn.functional.conv1d(flat, w.view(1, 1, -1).to(x.dtype))
-    return out.reshape(shp).movedim(-1, dim)
+    n = x.shape[dim]
+    wl = [float(v) for v in w.cpu()]
+    out = x * wl[rad]
+    for k in range(1, rad + 1):
+        out += wl[rad + k] * (torch.roll(x, k % n, dims=dim) + torch.roll(x, -k % n, dims=dim))
+    return out
 
 
 def grf(shape, corr_h: float, corr_v: float, gen: torch.Generator, device) -> torch.Tensor:
     """Unit-variance Gaussian random field on (nlev, nlat, nlon) or
     (nlat, nlon); correlation exp(-r^2/L^2) with L = corr_h cells
     horizontally and corr_v levels vertically."""
-    x = torch.randn(shape, generator=gen, dtype=torch.float32, device=device).double()
+    x = torch.randn(shape, generator=gen, dtype=torch.float32, device=device)
     var = 1.0
     wh = _kernel(corr_h / 2.0, device)
     x = _blur(x, -1, wh)
@@ -151,7 +150,7 @@ def grf(shape, corr_h: float, corr_v: float, gen: torch.Generator, device) -> to
         wv = _kernel(corr_v / 2.0, device)
         x = _blur(x, 0, wv)
         var *= float((wv ** 2).sum())
-    return x / math.sqrt(var)
+    return (x / math.sqrt(var)).double()
 
 
 def make_problem(nlon=64, nlat=64, nlev=16, nmem=20, nv3d=4, nv2d=1, nobs=2000,
@@ -159,7 +158,7 @@ def make_problem(nlon=64, nlat=64, nlev=16, nmem=20, nv3d=4, nv2d=1, nobs=2000,
                  alpha_rtpp=0.7, alpha_rtps=0.0, cov_infl_mul=1.0, real_kind=8,
                  seed=20261203, corr_h=8.0, corr_v=2.0, land_frac=0.0,
                  sigma_h_cells=3.0, sigma_v=2.0, dz=1.0, device="cpu",
-                 tracer=False, keep_truth=False) -> Problem:
+                 tracer=False, keep_truth=False, progress=None) -> Problem:
     """Synthetic problem.  obs_layout "random": nobs obs at uniform random
     nodes/levels with the element uniform over {U,V,T,S,SSH};
     "swath_profile": n_surface SST/SSH obs on 20-cell satellite-like stripes
@@ -218,6 +217,8 @@ def make_problem(nlon=64, nlat=64, nlev=16, nmem=20, nv3d=4, nv2d=1, nobs=2000,
     fcst3 = torch.empty((nv3d, nmem, nlev, nlat, nlon), dtype=rdt, device=dev)
     fcst2 = torch.empty((nv2d, nmem, nlat, nlon), dtype=rdt, device=dev)
     for v, nm in enumerate(names3):
+        if progress:
+            progress(f"ensemble variable {nm} ({v + 1}/{nv3d})")
         for m in range(nmem):
             e = grf((nlev, nlat, nlon), corr_h, corr_v, gen, dev)
             if nm == "TRACER":
