@@ -27,7 +27,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ld, row-major), d/e scratch of n doubles each.  On return d holds the
 // eigenvalues (unordered) and column j of Z the eigenvector of d[j].
 // Returns false if the QL iteration did not converge.
-__device__ bool eig_wave(int n, double* Z, int ld, double* d, double* e) {
+// Householder reduction of the symmetric n x n matrix in LDS Z (lower
+// triangle used) to tridiagonal form, accumulating Q in Z: on return d is
+// the diagonal, e[1..n-1] the sub-diagonal (e[0] = 0), Z = Q.
+__device__ void tridiag_wave(int n, double* Z, int ld, double* d, double* e) {
   const int lane = threadIdx.x & 63;
 #define Z_(r, c) Z[(r) * ld + (c)]
   // ---- Householder tridiagonalisation (lower triangle) ----
@@ -119,6 +122,14 @@ __device__ bool eig_wave(int n, double* Z, int ld, double* d, double* e) {
     if (lane == 0) Z_(i, i) = 1.0;
     __syncthreads();
   }
+#undef Z_
+}
+
+// Implicit-shift QL on (d, e) from tridiag_wave, rotations applied to the
+// rows of Z by the 64 lanes; the scalar recurrence runs in every lane.
+__device__ bool ql_wave(int n, double* Z, int ld, double* d, double* e) {
+  const int lane = threadIdx.x & 63;
+#define Z_(r, c) Z[(r) * ld + (c)]
   // ---- implicit QL ----
   for (int q = lane + 1; q < n; q += 64) e[q - 1] = e[q];
   __syncthreads();
@@ -202,6 +213,41 @@ __device__ bool eig_wave(int n, double* Z, int ld, double* d, double* e) {
   return ok;
 }
 
+__device__ bool eig_wave(int n, double* Z, int ld, double* d, double* e) {
+  tridiag_wave(n, Z, ld, d, e);
+  return ql_wave(n, Z, ld, d, e);
+}
+
+// Descending order (ties keep the reverse of DSYEV's ascending order),
+// negative eigenvalues set to 0 with their vectors (common.f90:1030-1043),
+// status bits for no convergence / all negative / non-finite.  lb gets the
+// n eigenvalues, Vb the eigenvectors with row pitch n.
+__device__ void finish_eig(int n, const double* Z, int ld, const double* d, double* lb, double* Vb, bool ok,
+                           int* status) {
+  const int lane = threadIdx.x & 63;
+  double dmax = -INFINITY;
+  bool finite = true;
+  for (int j = 0; j < n; ++j) {
+    dmax = fmax(dmax, d[j]);
+    finite = finite && isfinite(d[j]);
+  }
+  for (int j = lane; j < n && finite; j += 64) {
+    const double dj = d[j];
+    int rank = 0;
+    for (int c = 0; c < n; ++c) rank += (d[c] > dj) || (d[c] == dj && c > j);
+    const bool neg = dj < 0.0;
+    lb[rank] = neg ? 0.0 : dj;
+    for (int r = 0; r < n; ++r) Vb[r * n + rank] = neg ? 0.0 : Z[r * ld + j];
+  }
+  if (lane == 0) {
+    int st = 0;
+    if (!ok) st |= ST_EIG_NOCONV;
+    if (dmax < 0.0) st |= ST_EIG_NEG;
+    if (!finite) st |= ST_NONFINITE;
+    if (st) atomicOr(status, st);
+  }
+}
+
 // One matrix per 64-lane workgroup.  Slot b starts at G + b*K*K; its size
 // is K, or in the dual form the point's local-obs count when below K.
 // Dynamic LDS: K*(K+1) + 2K doubles.
@@ -238,30 +284,7 @@ __global__ void __launch_bounds__(64) k_eig_fp64(int K, const double* __restrict
   } else {
     ok = eig_wave(n, Z, ld, d, e);
   }
-  // descending order; ties keep the reverse of DSYEV's ascending order
-  double* lb = lam + (long long)blockIdx.x * K;
-  double* Vb = V + (long long)blockIdx.x * K * K;
-  double dmax = -INFINITY;
-  bool finite = true;
-  for (int j = 0; j < n; ++j) {
-    dmax = fmax(dmax, d[j]);
-    finite = finite && isfinite(d[j]);
-  }
-  for (int j = lane; j < n && finite; j += 64) {
-    const double dj = d[j];
-    int rank = 0;
-    for (int c = 0; c < n; ++c) rank += (d[c] > dj) || (d[c] == dj && c > j);
-    const bool neg = dj < 0.0;
-    lb[rank] = neg ? 0.0 : dj;
-    for (int r = 0; r < n; ++r) Vb[r * n + rank] = neg ? 0.0 : Z[r * ld + j];
-  }
-  if (lane == 0) {
-    int st = 0;
-    if (!ok) st |= ST_EIG_NOCONV;
-    if (dmax < 0.0) st |= ST_EIG_NEG;
-    if (!finite) st |= ST_NONFINITE;
-    if (st) atomicOr(status, st);
-  }
+  finish_eig(n, Z, ld, d, lam + (long long)blockIdx.x * K, V + (long long)blockIdx.x * K * K, ok, status);
 }
 
 hipError_t launch_eig_fp64(int K, int na, const double* G, double* lam, double* V, int* status, int dual,
@@ -274,6 +297,233 @@ hipError_t launch_eig_fp64(int K, int na, const double* G, double* lam, double* 
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_eig_fp64, dim3(na), dim3(64), lds, s, K, G, lam, V, status, dual, nloc, act_p, a0);
+  return hipGetLastError();
+}
+
+
+// ------------------------------------------------------------------------
+// Three-phase form of the same solve, used by the analysis pipeline:
+//   1. k_eig_tridiag  one wavefront per matrix: tridiag_wave in LDS, Q out;
+//   2. k_eig_ql       one lane per matrix: the implicit-QL recurrence on
+//                     (d, e) of 64 matrices per wavefront at once, every
+//                     Givens rotation (c, s, column) appended to a log;
+//   3. k_eig_apply    one wavefront per matrix: the logged rotations applied
+//                     to the rows of Q in LDS, then ordering as finish_eig.
+// The scalar recurrence, which is a long dependent chain, thus runs once
+// per matrix instead of once per lane.  A matrix whose log would overflow
+// is solved whole by eig_wave in phase 3.
+
+constexpr int kSweepStart = 1 << 20;   // flag on the first rotation of a sweep
+
+struct EigSlots {
+  int K;  // slot pitch: K*K doubles per matrix in G/Q, K in D/E/lam
+  int dual;
+  const int32_t* nloc;
+  const int32_t* act_p;
+  int a0;
+  __device__ int size(int b) const {
+    if (!dual) return K;
+    const int nl = nloc[act_p[a0 + b]];
+    return nl < K ? nl : K;
+  }
+};
+
+struct RotLog {
+  double* c;
+  double* s;
+  int32_t* col;
+  int32_t* count;  // per matrix; -1 overflow (phase 3 falls back), -2 no convergence
+  long long cap;   // rotations per matrix
+};
+
+__device__ __forceinline__ void load_sym(int n, const double* Gb, double* Z, int ld) {
+  for (int q = threadIdx.x; q < n * n; q += 64) {
+    const int r = q / n, c = q - r * n;
+    Z[r * ld + c] = r <= c ? Gb[r * n + c] : Gb[c * n + r];
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) k_eig_tridiag(EigSlots es, const double* __restrict__ G,
+                                                    double* __restrict__ Q, double* __restrict__ D,
+                                                    double* __restrict__ E) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, n = es.size(b), ld = n + 1, K = es.K;
+  double* Z = lds;
+  double* d = Z + n * ld;
+  double* e = d + n;
+  load_sym(n, G + (long long)b * K * K, Z, ld);
+  if (n == 1) {
+    if (threadIdx.x == 0) {
+      d[0] = Z[0];
+      e[0] = 0.0;
+      Z[0] = 1.0;
+    }
+    __syncthreads();
+  } else {
+    tridiag_wave(n, Z, ld, d, e);
+  }
+  double* Qb = Q + (long long)b * K * K;
+  for (int q = threadIdx.x; q < n * n; q += 64) {
+    const int r = q / n, c = q - r * n;
+    Qb[q] = Z[r * ld + c];
+  }
+  for (int q = threadIdx.x; q < n; q += 64) {
+    D[(long long)b * K + q] = d[q];
+    E[(long long)b * K + q] = e[q];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_eig_ql(int na, EigSlots es, double* __restrict__ D, double* __restrict__ E,
+                                                RotLog lg) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= na) return;
+  const int n = es.size(b);
+  double* d = D + (long long)b * es.K;
+  double* e = E + (long long)b * es.K;
+  double* lc = lg.c + (long long)b * lg.cap;
+  double* ls = lg.s + (long long)b * lg.cap;
+  int32_t* li = lg.col + (long long)b * lg.cap;
+  long long nr = 0;
+  bool overflow = false, ok = true;
+  for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+  e[n - 1] = 0.0;
+  for (int l = 0; l < n && ok; ++l) {
+    int sweeps = 0;
+    for (;;) {
+      int m;
+      for (m = l; m < n - 1; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (m == l) break;
+      if (++sweeps > 60) {
+        ok = false;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      double s = 1.0, c = 1.0, p = 0.0;
+      bool deflated = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i], bb = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          deflated = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * bb;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - bb;
+        if (nr < lg.cap) {
+          lc[nr] = c;
+          ls[nr] = s;
+          li[nr] = i | (i == m - 1 ? kSweepStart : 0);
+        } else {
+          overflow = true;
+        }
+        ++nr;
+      }
+      if (deflated) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  lg.count[b] = !ok ? -2 : (overflow ? -1 : (int)nr);
+}
+
+__global__ void __launch_bounds__(64) k_eig_apply(EigSlots es, const double* __restrict__ G,
+                                                  const double* __restrict__ Q, const double* __restrict__ D,
+                                                  const RotLog lg, double* __restrict__ lam, double* __restrict__ V,
+                                                  int* __restrict__ status) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, n = es.size(b), ld = n + 1, K = es.K;
+  const int lane = threadIdx.x;
+  double* Z = lds;
+  double* d = Z + n * ld;
+  double* e = d + n;
+  const int cnt = lg.count[b];
+  bool ok = true;
+  if (cnt == -1) {
+    // rotation log overflowed: solve this matrix whole
+    load_sym(n, G + (long long)b * K * K, Z, ld);
+    ok = eig_wave(n, Z, ld, d, e);
+  } else {
+    ok = cnt >= 0;
+    const double* Qb = Q + (long long)b * K * K;
+    for (int q = lane; q < n * n; q += 64) {
+      const int r = q / n, c = q - r * n;
+      Z[r * ld + c] = Qb[q];
+    }
+    for (int q = lane; q < n; q += 64) d[q] = D[(long long)b * K + q];
+    __syncthreads();
+    const double* __restrict__ lc = lg.c + (long long)b * lg.cap;
+    const double* __restrict__ ls = lg.s + (long long)b * lg.cap;
+    const int32_t* __restrict__ li = lg.col + (long long)b * lg.cap;
+    const bool r0 = lane < n, r1 = lane + 64 < n;
+    double* z0 = Z + lane * ld;
+    double* z1 = Z + (lane + 64) * ld;
+    int j = 0;
+    while (j < cnt) {
+      int i = li[j] & (kSweepStart - 1);
+      // column i+1 of each row carried in registers along the sweep
+      double carry0 = r0 ? z0[i + 1] : 0.0;
+      double carry1 = r1 ? z1[i + 1] : 0.0;
+      for (;;) {
+        const double c = lc[j], s = ls[j];
+        if (r0) {
+          const double zi = z0[i];
+          z0[i + 1] = s * zi + c * carry0;
+          carry0 = c * zi - s * carry0;
+        }
+        if (r1) {
+          const double zi = z1[i];
+          z1[i + 1] = s * zi + c * carry1;
+          carry1 = c * zi - s * carry1;
+        }
+        ++j;
+        if (j >= cnt || (li[j] & kSweepStart)) break;
+        i = li[j];
+      }
+      if (r0) z0[i] = carry0;
+      if (r1) z1[i] = carry1;
+    }
+    __syncthreads();
+  }
+  finish_eig(n, Z, ld, d, lam + (long long)b * K, V + (long long)b * K * K, ok, status);
+}
+
+// Rotations logged per matrix (capacity); a typical implicit-QL solve needs
+// about 1.3 n^2, the rest is the fallback's business.
+long long eig_log_capacity(int K) { return 2LL * K * K + 64; }
+
+hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D, double* E, double* lc, double* ls,
+                       int32_t* li, int32_t* cnt, double* lam, double* V, int* status, int dual,
+                       const int32_t* nloc, const int32_t* act_p, int a0, hipStream_t st) {
+  if (na <= 0) return hipSuccess;
+  EigSlots es{K, dual, nloc, act_p, a0};
+  RotLog lg{lc, ls, li, cnt, eig_log_capacity(K)};
+  const size_t lds = sizeof(double) * ((size_t)K * (K + 1) + 2 * (size_t)K);
+  if (lds > 65536) {
+    hipError_t e1 = hipFuncSetAttribute((const void*)k_eig_tridiag, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds);
+    hipError_t e2 = hipFuncSetAttribute((const void*)k_eig_apply, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds);
+    if (e1 != hipSuccess) return e1;
+    if (e2 != hipSuccess) return e2;
+  }
+  hipLaunchKernelGGL(k_eig_tridiag, dim3(na), dim3(64), lds, st, es, G, Q, D, E);
+  hipLaunchKernelGGL(k_eig_ql, dim3((na + 255) / 256), dim3(256), 0, st, na, es, D, E, lg);
+  hipLaunchKernelGGL(k_eig_apply, dim3(na), dim3(64), lds, st, es, G, Q, D, lg, lam, V, status);
   return hipGetLastError();
 }
 

@@ -262,6 +262,13 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   HIP_TRY(h->V.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
   HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+  const long long cap = eig_log_capacity(K);
+  HIP_TRY(h->eD.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+  HIP_TRY(h->eE.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+  HIP_TRY(h->logc.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
+  HIP_TRY(h->logs.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
+  HIP_TRY(h->logi.ensure(sizeof(int32_t) * (size_t)(maxA + 1) * cap));
+  HIP_TRY(h->logn.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
 
   if (timing) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
   for (long long c = 0; c < nch; ++c) {
@@ -283,8 +290,10 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     }
     HIP_TRY(mark(e0 + 2));
     if (na > 0) {
-      HIP_TRY(launch_eig_fp64(K, na, h->G.as<double>(), h->lam.as<double>(), h->V.as<double>(),
-                              h->status.as<int>(), dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0, s));
+      HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
+                          h->logc.as<double>(), h->logs.as<double>(), h->logi.as<int32_t>(), h->logn.as<int32_t>(),
+                          h->lam.as<double>(), h->V.as<double>(), h->status.as<int>(), dual,
+                          h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0, s));
     }
     HIP_TRY(mark(e0 + 3));
     HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,

@@ -53,6 +53,7 @@ struct letkf_handle {
   hipStream_t stream = nullptr;
   DevBuf nloc, loff, actidx, act_p, scan_tmp, yT, lobs, lrloc, G, lam, V, bvec, status, bounds;
   DevBuf nrow, bin_tmp;
+  DevBuf eD, eE, logc, logs, logi, logn;  // three-phase eigensolve workspaces
   // staging for the host-pointer entry point
   DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
   letkf_stats_t stats{};
@@ -68,7 +69,7 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();

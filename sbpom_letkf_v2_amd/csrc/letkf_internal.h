@@ -106,6 +106,16 @@ hipError_t launch_eig_fp64(int K, int na, const double* G, double* lam,
                            const int32_t* nloc, const int32_t* act_p, int a0,
                            hipStream_t s);
 
+// Three-phase variant (tridiagonalise / QL with rotation log / apply);
+// Q, D, E, lc, ls, li, cnt are per-slot workspaces (K*K, K, K, cap, cap,
+// cap, 1 per slot, cap = eig_log_capacity(K)).
+long long eig_log_capacity(int K);
+hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
+                       double* E, double* lc, double* ls, int32_t* li,
+                       int32_t* cnt, double* lam, double* V, int* status,
+                       int dual, const int32_t* nloc, const int32_t* act_p,
+                       int a0, hipStream_t s);
+
 // Weights + ensemble transform + RTPP/RTPS + round_off for points [p0,p1).
 template <typename T>
 hipError_t launch_transform(const KParams& kp, const Tables<T>& tb,
