@@ -165,6 +165,34 @@ __global__ void k_gather_bounds(int nb, const long long* pts, const int64_t* lof
   }
 }
 
+// max nobsl and the number of points with 0 < nobsl < K (solved in the
+// nobsl x nobsl form by the batched form).
+__global__ void k_point_stats(long long P, const int32_t* nloc, int K, unsigned long long* out) {
+  __shared__ int smax[256];
+  __shared__ unsigned long long sdual[256];
+  int mx = 0;
+  unsigned long long du = 0;
+  for (long long p = blockIdx.x * 256LL + threadIdx.x; p < P; p += 256LL * gridDim.x) {
+    const int n = nloc[p];
+    mx = n > mx ? n : mx;
+    du += (n > 0 && n < K);
+  }
+  smax[threadIdx.x] = mx;
+  sdual[threadIdx.x] = du;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
+      sdual[threadIdx.x] += sdual[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(out, (unsigned long long)smax[0]);
+    atomicAdd(out + 1, sdual[0]);
+  }
+}
+
 template <typename T>
 static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2, T* a3,
                       T* a2, hipStream_t s) {
@@ -239,16 +267,22 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   const long long nch = (P + PC - 1) / PC;
   std::vector<long long> bpts(nch + 1);
   for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
-  HIP_TRY(h->bounds.ensure(sizeof(long long) * 3 * (nch + 1)));
+  HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 2)));
   long long* d_pts = h->bounds.as<long long>();
   long long* d_out = d_pts + (nch + 1);
+  unsigned long long* d_pst = (unsigned long long*)(d_out + 2 * (nch + 1));
+  HIP_TRY(hipMemsetAsync(d_pst, 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_point_stats, dim3(1024), dim3(256), 0, s, P, h->nloc.as<int32_t>(), K, d_pst);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(d_pts, bpts.data(), sizeof(long long) * (nch + 1), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_gather_bounds, dim3((unsigned)((nch + 1 + 255) / 256)), dim3(256), 0, s, (int)(nch + 1),
                      d_pts, h->loff.as<int64_t>(), h->actidx.as<int32_t>(), d_out);
   HIP_TRY(hipGetLastError());
-  std::vector<long long> bv(2 * (nch + 1));
-  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * 2 * (nch + 1), hipMemcpyDeviceToHost, s));
+  std::vector<long long> bv(2 * (nch + 1) + 2);
+  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * (2 * (nch + 1) + 2), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  st.max_nobsl = bv[2 * (nch + 1)];
+  st.dual_points = p.form == LETKF_FORM_BATCHED ? bv[2 * (nch + 1) + 1] : 0;
   long long maxL = 0, maxA = 0;
   for (long long c = 0; c < nch; ++c) {
     maxL = std::max(maxL, bv[2 * (c + 1)] - bv[2 * c]);
