@@ -119,6 +119,32 @@ def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2):
                        f"FP64 C oracle with {cores} OpenMP threads, {dt:.1f} s"), max(err3, err2), npt
 
 
+def field_errors(a3, a2, r3, r2, nv3, nv2, K):
+    """Per-variable max and RMS error of the analysis ensemble mean and
+    spread against the FP64 analysis, relative to the max |FP64 mean| and
+    to the max FP64 spread of that variable."""
+    import torch
+    out = {}
+    worst_mean = worst_sprd = 0.0
+    names = ["U", "V", "T", "S", "TRACER"][:nv3] + ["SSH"][:nv2]
+    for v in range(nv3 + nv2):
+        if v < nv3:
+            x, r = a3.reshape(nv3, K, -1)[v], r3.reshape(nv3, K, -1)[v]
+        else:
+            x, r = a2.reshape(nv2, K, -1)[v - nv3], r2.reshape(nv2, K, -1)[v - nv3]
+        xm, rm = x.mean(0), r.mean(0)
+        xs, rs = x.std(0), r.std(0)
+        dm, ds = (xm - rm).abs(), (xs - rs).abs()
+        sm, ss = rm.abs().max().item(), rs.max().item()
+        e = {"mean_max": dm.max().item() / sm, "mean_rms": dm.pow(2).mean().sqrt().item() / sm,
+             "sprd_max": ds.max().item() / ss, "sprd_rms": ds.pow(2).mean().sqrt().item() / ss}
+        out[names[v] if v < len(names) else f"v{v}"] = e
+        worst_mean = max(worst_mean, e["mean_max"])
+        worst_sprd = max(worst_sprd, e["sprd_max"])
+    out["summary"] = {"mean_max_rel": worst_mean, "sprd_max_rel": worst_sprd}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -146,14 +172,16 @@ def main():
     a3, a2, sec, stage, st = run_form(prd, args.precision, args.form, args.steps, args.warmup)
     log(f"  {sec * 1e3:.1f} ms/step, stages {stage}")
     result_extra = {}
-    if not args.no_plain and (args.precision, args.form) != ("fp64", "plain"):
+    ref3 = ref2 = None                      # FP64 analysis the errors are measured against
+    if (args.precision, args.form) != ("fp64", "plain") and not args.no_plain:
         log("timing plain FP64 form")
-        p3, p2, psec, pstage, pst = run_form(prd, "fp64", "plain", max(1, args.steps // 2), 1)
-        d3 = (a3 - p3).abs()
-        result_extra["plain_fp64"] = {"value": P / psec, "ms_per_step": psec * 1e3,
-                                      "stage_ms": pstage}
+        ref3, ref2, psec, pstage, pst = run_form(prd, "fp64", "plain", max(1, args.steps // 2), 1)
+        result_extra["plain_fp64"] = {"value": P / psec, "ms_per_step": psec * 1e3, "stage_ms": pstage}
         result_extra["speedup_vs_plain_fp64"] = psec / sec
-        del p3, p2
+        result_extra["errors_vs_fp64"] = field_errors(a3, a2, ref3, ref2, nv3, nv2, K)
+        log(f"  speedup {psec / sec:.2f}x; errors {result_extra['errors_vs_fp64']['summary']}")
+    elif args.precision == "fp64":
+        ref3, ref2 = a3, a2
     # --- roofline of the dominant kernel (stage times measured with HIP events) ---
     L = st["local_obs"]
     na = st["active_points"]
@@ -192,9 +220,8 @@ def main():
             fcst2d=torch.zeros((max(nv2, 1), K, 1), dtype=torch.float64)).numpy()
         opr = po.Problem(params=dict(pn_tabs.params),
                          **{k: getattr(pn_tabs, k) for k in synth.Problem.ARRAYS})
-        ref_a3 = a3 if (args.precision, args.form) == ("fp64", "plain") else None
-        if ref_a3 is not None:
-            cpu, parity, npt = cpu_baseline(opr, prd, a3, a2, args.cpu_seconds, K, nv3, nv2)
+        if ref3 is not None:
+            cpu, parity, npt = cpu_baseline(opr, prd, ref3, ref2, args.cpu_seconds, K, nv3, nv2)
             log(f"  cpu {cpu['value']:.0f} pts/s on {cpu['cores']} cores; GPU vs oracle max rel err "
                 f"{parity:.2e} over {npt} points")
     out = {

@@ -17,10 +17,35 @@
 
 namespace letkf {
 
+// Wavefront sum of a double with DPP moves (no LDS traffic): butterfly
+// within quads, half-row and row mirrors, then the row broadcasts 15 and 31
+// fold the four rows into lane 63, which readlane hands to every lane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, ROWMASK, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_d<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_d<0x140, 0xF>(v);  // row_mirror
+  v += dpp_d<0x142, 0xA>(v);  // row_bcast15 -> rows 1, 3
+  v += dpp_d<0x143, 0xC>(v);  // row_bcast31 -> rows 2, 3
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)x, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The eigensolver workgroups are single wavefronts: a wavefront's LDS
+// accesses execute in program order, so ordering the compiler is enough.
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Eigen-decomposition of one n x n symmetric matrix held in LDS Z (pitch
@@ -41,19 +66,19 @@ __device__ void tridiag_wave(int n, double* Z, int ld, double* d, double* e) {
         e[i] = Z_(i, 0);
         d[i] = 0.0;
       }
-      __syncthreads();
+      wsync();
       continue;
     }
     double part = 0.0;
     for (int c = lane; c <= l; c += 64) part += fabs(Z_(i, c));
     const double scale = wave_sum(part);
     if (scale == 0.0) {
-      __syncthreads();
+      wsync();
       if (lane == 0) {
         e[i] = Z_(i, l);
         d[i] = 0.0;
       }
-      __syncthreads();
+      wsync();
       continue;
     }
     part = 0.0;
@@ -63,64 +88,79 @@ __device__ void tridiag_wave(int n, double* Z, int ld, double* d, double* e) {
       part += v * v;
     }
     double h = wave_sum(part);
-    __syncthreads();
+    wsync();
     const double f = Z_(i, l);
     const double g = f >= 0.0 ? -sqrt(h) : sqrt(h);
     h -= f * g;
-    __syncthreads();
+    wsync();
     if (lane == 0) {
       e[i] = scale * g;
       Z_(i, l) = f - g;
     }
-    __syncthreads();
+    wsync();
     // p = A u / h into e[0..l]
     part = 0.0;
     for (int r = lane; r <= l; r += 64) {
       Z_(r, i) = Z_(i, r) / h;
-      double acc = 0.0;
-      for (int c = 0; c <= l; ++c) {
-        const double arc = c <= r ? Z_(r, c) : Z_(c, r);
-        acc += arc * Z_(i, c);
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int c = 0;
+      for (; c + 3 <= l; c += 4) {
+        a0 += (c <= r ? Z_(r, c) : Z_(c, r)) * Z_(i, c);
+        a1 += (c + 1 <= r ? Z_(r, c + 1) : Z_(c + 1, r)) * Z_(i, c + 1);
+        a2 += (c + 2 <= r ? Z_(r, c + 2) : Z_(c + 2, r)) * Z_(i, c + 2);
+        a3 += (c + 3 <= r ? Z_(r, c + 3) : Z_(c + 3, r)) * Z_(i, c + 3);
       }
+      for (; c <= l; ++c) a0 += (c <= r ? Z_(r, c) : Z_(c, r)) * Z_(i, c);
+      const double acc = (a0 + a1) + (a2 + a3);
       const double pr = acc / h;
       e[r] = pr;
       part += pr * Z_(i, r);
     }
     const double kk = wave_sum(part) / (h + h);
-    __syncthreads();
+    wsync();
     for (int r = lane; r <= l; r += 64) e[r] = e[r] - kk * Z_(i, r);
-    __syncthreads();
+    wsync();
     for (int r = lane; r <= l; r += 64) {
       const double ur = Z_(i, r), qr = e[r];
+#pragma unroll 4
       for (int c = 0; c <= r; ++c) Z_(r, c) -= ur * e[c] + qr * Z_(i, c);
     }
     if (lane == 0) d[i] = h;
-    __syncthreads();
+    wsync();
   }
   if (lane == 0) {
     d[0] = 0.0;
     e[0] = 0.0;
   }
-  __syncthreads();
+  wsync();
   // ---- accumulate Q ----
   for (int i = 0; i < n; ++i) {
     const int l = i - 1;
     if (d[i] != 0.0) {
       for (int c = lane; c <= l; c += 64) {
-        double g = 0.0;
-        for (int r = 0; r <= l; ++r) g += Z_(i, r) * Z_(r, c);
-        for (int r = 0; r <= l; ++r) Z_(r, c) -= g * Z_(r, i);
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+        int r = 0;
+        for (; r + 3 <= l; r += 4) {
+          g0 += Z_(i, r) * Z_(r, c);
+          g1 += Z_(i, r + 1) * Z_(r + 1, c);
+          g2 += Z_(i, r + 2) * Z_(r + 2, c);
+          g3 += Z_(i, r + 3) * Z_(r + 3, c);
+        }
+        for (; r <= l; ++r) g0 += Z_(i, r) * Z_(r, c);
+        const double g = (g0 + g1) + (g2 + g3);
+#pragma unroll 4
+        for (int rr = 0; rr <= l; ++rr) Z_(rr, c) -= g * Z_(rr, i);
       }
     }
-    __syncthreads();
+    wsync();
     if (lane == 0) d[i] = Z_(i, i);
-    __syncthreads();
+    wsync();
     for (int c = lane; c <= l; c += 64) {
       Z_(c, i) = 0.0;
       Z_(i, c) = 0.0;
     }
     if (lane == 0) Z_(i, i) = 1.0;
-    __syncthreads();
+    wsync();
   }
 #undef Z_
 }
@@ -132,9 +172,9 @@ __device__ bool ql_wave(int n, double* Z, int ld, double* d, double* e) {
 #define Z_(r, c) Z[(r) * ld + (c)]
   // ---- implicit QL ----
   for (int q = lane + 1; q < n; q += 64) e[q - 1] = e[q];
-  __syncthreads();
+  wsync();
   if (lane == 0) e[n - 1] = 0.0;
-  __syncthreads();
+  wsync();
   bool ok = true;
   for (int l = 0; l < n; ++l) {
     int sweeps = 0;
@@ -160,7 +200,7 @@ __device__ bool ql_wave(int n, double* Z, int ld, double* d, double* e) {
       for (int i = m - 1; i >= l; --i) {
         const double ei = e[i], f = s * ei, b = c * ei;
         r = hypot(f, g);
-        __syncthreads();
+        wsync();
         if (lane == 0) e[i + 1] = r;
         if (r == 0.0) {
           if (lane == 0) {
@@ -177,7 +217,7 @@ __device__ bool ql_wave(int n, double* Z, int ld, double* d, double* e) {
         g = d[i + 1] - p;
         r = (d[i] - g) * s + 2.0 * c * b;
         p = s * r;
-        __syncthreads();
+        wsync();
         if (lane == 0) d[i + 1] = g + p;
         g = c * r - b;
         if (lane < n) {
@@ -193,22 +233,22 @@ __device__ bool ql_wave(int n, double* Z, int ld, double* d, double* e) {
       }
       if (!ok) break;
       if (deflated) {
-        __syncthreads();
+        wsync();
         continue;
       }
       if (lane < n) Z_(lane, l) = carry0;
       if (lane + 64 < n) Z_(lane + 64, l) = carry1;
-      __syncthreads();
+      wsync();
       if (lane == 0) {
         d[l] -= p;
         e[l] = g;
         e[m] = 0.0;
       }
-      __syncthreads();
+      wsync();
     }
     if (!ok) break;
   }
-  __syncthreads();
+  wsync();
 #undef Z_
   return ok;
 }
@@ -272,14 +312,14 @@ __global__ void __launch_bounds__(64) k_eig_fp64(int K, const double* __restrict
     const int r = q / n, c = q - r * n;
     Z[r * ld + c] = r <= c ? Gb[r * n + c] : Gb[c * n + r];
   }
-  __syncthreads();
+  wsync();
   bool ok;
   if (n == 1) {
     if (lane == 0) {
       d[0] = Z[0];
       Z[0] = 1.0;
     }
-    __syncthreads();
+    wsync();
     ok = true;
   } else {
     ok = eig_wave(n, Z, ld, d, e);
@@ -470,30 +510,67 @@ __global__ void __launch_bounds__(64) k_eig_apply(EigSlots es, const double* __r
     const double* __restrict__ ls = lg.s + (long long)b * lg.cap;
     const int32_t* __restrict__ li = lg.col + (long long)b * lg.cap;
     const bool r0 = lane < n, r1 = lane + 64 < n;
-    double* z0 = Z + lane * ld;
-    double* z1 = Z + (lane + 64) * ld;
-    int j = 0;
-    while (j < cnt) {
-      int i = li[j] & (kSweepStart - 1);
-      // column i+1 of each row carried in registers along the sweep
-      double carry0 = r0 ? z0[i + 1] : 0.0;
-      double carry1 = r1 ? z1[i + 1] : 0.0;
-      for (;;) {
-        const double c = lc[j], s = ls[j];
-        if (r0) {
-          const double zi = z0[i];
-          z0[i + 1] = s * zi + c * carry0;
-          carry0 = c * zi - s * carry0;
+    double* z0 = Z + (r0 ? lane : 0) * ld;
+    double* z1 = Z + (r1 ? lane + 64 : 0) * ld;
+    // Rotations arrive 64 at a time (one coalesced load per array, lane t
+    // holding rotation j0+t) and are broadcast with readlane, so the chain
+    // per rotation is LDS + FMA only.  Column i+1 of each row is carried in
+    // registers along a sweep; the next column is read before this one is
+    // stored (i-1 is never written by rotation i).
+    int i = -1;                      // column of the last applied rotation
+    double carry0 = 0.0, carry1 = 0.0, zp0 = 0.0, zp1 = 0.0;
+    bool have_pref = false;
+    for (int j0 = 0; j0 < cnt; j0 += 64) {
+      const int nb = min(64, cnt - j0);
+      const double cv = lane < nb ? lc[j0 + lane] : 0.0;
+      const double sv = lane < nb ? ls[j0 + lane] : 0.0;
+      const int iv = lane < nb ? li[j0 + lane] : 0;
+      have_pref = false;
+      for (int t = 0; t < nb; ++t) {
+        const int code = __builtin_amdgcn_readlane(iv, t);
+        const double c = __longlong_as_double(
+            ((long long)__builtin_amdgcn_readlane((int)(__double_as_longlong(cv) >> 32), t) << 32) |
+            (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(cv), t));
+        const double s = __longlong_as_double(
+            ((long long)__builtin_amdgcn_readlane((int)(__double_as_longlong(sv) >> 32), t) << 32) |
+            (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(sv), t));
+        const int ic = code & (kSweepStart - 1);
+        if (code & kSweepStart) {
+          if (i >= 0) {              // close the previous sweep
+            if (r0) z0[i] = carry0;
+            if (r1) z1[i] = carry1;
+          }
+          carry0 = r0 ? z0[ic + 1] : 0.0;
+          carry1 = r1 ? z1[ic + 1] : 0.0;
         }
-        if (r1) {
-          const double zi = z1[i];
-          z1[i + 1] = s * zi + c * carry1;
-          carry1 = c * zi - s * carry1;
+        i = ic;
+        double zi0, zi1;
+        if (have_pref && !(code & kSweepStart)) {
+          zi0 = zp0;
+          zi1 = zp1;
+        } else {
+          zi0 = r0 ? z0[i] : 0.0;
+          zi1 = r1 ? z1[i] : 0.0;
         }
-        ++j;
-        if (j >= cnt || (li[j] & kSweepStart)) break;
-        i = li[j];
+        if (r0) z0[i + 1] = s * zi0 + c * carry0;
+        if (r1) z1[i + 1] = s * zi1 + c * carry1;
+        carry0 = c * zi0 - s * carry0;
+        carry1 = c * zi1 - s * carry1;
+        // read the next rotation's column now when it continues this sweep
+        // (column i-1, untouched by rotation i)
+        have_pref = false;
+        if (t + 1 < nb) {
+          const int code2 = __builtin_amdgcn_readlane(iv, t + 1);
+          if (!(code2 & kSweepStart)) {
+            const int i2 = code2 & (kSweepStart - 1);
+            zp0 = r0 ? z0[i2] : 0.0;
+            zp1 = r1 ? z1[i2] : 0.0;
+            have_pref = true;
+          }
+        }
       }
+    }
+    if (i >= 0) {
       if (r0) z0[i] = carry0;
       if (r1) z1[i] = carry1;
     }
