@@ -604,4 +604,6 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D, dou
   return hipGetLastError();
 }
 
+#include "eigreg.inc"
+
 }  // namespace letkf

@@ -193,6 +193,33 @@ __global__ void k_point_stats(long long P, const int32_t* nloc, int K, unsigned 
   }
 }
 
+// Size buckets of the register eigensolve: problem size s (K, or nobsl < K
+// in the dual form) goes to bucket (s-1)/16, i.e. nb = 16, 32, 48, 64.
+constexpr int kNQ = 4;
+
+__device__ __forceinline__ int slot_size(int n, int K, int dual) { return dual ? (n < K ? n : K) : K; }
+
+__global__ void k_bucket_count(long long P, const int32_t* nloc, int K, int dual, long long PC, int32_t* hist) {
+  for (long long p = blockIdx.x * 256LL + threadIdx.x; p < P; p += 256LL * gridDim.x) {
+    const int n = nloc[p];
+    if (n == 0) continue;
+    const int q = (slot_size(n, K, dual) - 1) / 16;
+    atomicAdd(&hist[(p / PC) * kNQ + q], 1);
+  }
+}
+
+__global__ void k_bucket_scatter(long long P, const int32_t* nloc, const int32_t* actidx, int K, int dual,
+                                 long long PC, int32_t* cursor, int32_t* list) {
+  for (long long p = blockIdx.x * 256LL + threadIdx.x; p < P; p += 256LL * gridDim.x) {
+    const int n = nloc[p];
+    if (n == 0) continue;
+    const long long c = p / PC;
+    const int q = (slot_size(n, K, dual) - 1) / 16;
+    const int pos = atomicAdd(&cursor[c * kNQ + q], 1);
+    list[pos] = actidx[p] - actidx[c * PC];
+  }
+}
+
 template <typename T>
 static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2, T* a3,
                       T* a2, hipStream_t s) {
@@ -290,18 +317,48 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   }
   st.local_obs = bv[2 * nch];
   st.active_points = bv[2 * nch + 1];
+  // register eigensolve for K <= 64: bucket the active slots of every chunk
+  // by problem size; bucket (c, q) occupies list[boff[c*kNQ+q] ...)
+  const bool reg_eig = K <= 64;
+  std::vector<int32_t> hist, boff;
+  if (reg_eig) {
+    const size_t nh = (size_t)nch * kNQ;
+    HIP_TRY(h->bhist.ensure(sizeof(int32_t) * nh));
+    HIP_TRY(h->bcur.ensure(sizeof(int32_t) * nh));
+    HIP_TRY(h->blist.ensure(sizeof(int32_t) * (size_t)(st.active_points + 1)));
+    HIP_TRY(hipMemsetAsync(h->bhist.ptr, 0, sizeof(int32_t) * nh, s));
+    hipLaunchKernelGGL(k_bucket_count, dim3(2048), dim3(256), 0, s, P, h->nloc.as<int32_t>(), K, dual, PC,
+                       h->bhist.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    hist.resize(nh);
+    boff.resize(nh);
+    HIP_TRY(hipMemcpyAsync(hist.data(), h->bhist.ptr, sizeof(int32_t) * nh, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int32_t run = 0;
+    for (size_t q = 0; q < nh; ++q) {
+      boff[q] = run;
+      run += hist[q];
+    }
+    HIP_TRY(hipMemcpyAsync(h->bcur.ptr, boff.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(2048), dim3(256), 0, s, P, h->nloc.as<int32_t>(),
+                       h->actidx.as<int32_t>(), K, dual, PC, h->bcur.as<int32_t>(), h->blist.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    // the host copy of boff must stay alive until the copy has run
+    HIP_TRY(hipStreamSynchronize(s));
+  }
   HIP_TRY(h->lobs.ensure(sizeof(int32_t) * (maxL + 1)));
   HIP_TRY(h->lrloc.ensure(sizeof(double) * (maxL + 1)));
   HIP_TRY(h->G.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
   HIP_TRY(h->V.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
   HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
-  const long long cap = eig_log_capacity(K);
+  const long long cap = reg_eig ? 2 * eigr_rot_capacity(K) : eig_log_capacity(K);
+  const long long hcap = reg_eig ? 2 * eigr_sweep_capacity(K) : eig_log_capacity(K);
   HIP_TRY(h->eD.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->eE.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->logc.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
-  HIP_TRY(h->logs.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
-  HIP_TRY(h->logi.ensure(sizeof(int32_t) * (size_t)(maxA + 1) * cap));
+  if (!reg_eig) HIP_TRY(h->logs.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
+  HIP_TRY(h->logi.ensure(sizeof(int32_t) * (size_t)(maxA + 1) * hcap));
   HIP_TRY(h->logn.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
 
   if (timing) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
@@ -324,10 +381,22 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     }
     HIP_TRY(mark(e0 + 2));
     if (na > 0) {
-      HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
-                          h->logc.as<double>(), h->logs.as<double>(), h->logi.as<int32_t>(), h->logn.as<int32_t>(),
-                          h->lam.as<double>(), h->V.as<double>(), h->status.as<int>(), dual,
-                          h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0, s));
+      if (reg_eig) {
+        for (int q = 0; q < kNQ; ++q) {
+          const int cnt = hist[(size_t)c * kNQ + q];
+          if (cnt == 0) continue;
+          HIP_TRY(launch_eigr(16 * (q + 1), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
+                              h->blist.as<int32_t>() + boff[(size_t)c * kNQ + q], cnt, h->G.as<double>(),
+                              h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(), h->logc.as<double>(),
+                              h->logi.as<int32_t>(), h->logn.as<int32_t>(), h->lam.as<double>(),
+                              h->V.as<double>(), h->status.as<int>(), s));
+        }
+      } else {
+        HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
+                            h->logc.as<double>(), h->logs.as<double>(), h->logi.as<int32_t>(),
+                            h->logn.as<int32_t>(), h->lam.as<double>(), h->V.as<double>(), h->status.as<int>(),
+                            dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0, s));
+      }
     }
     HIP_TRY(mark(e0 + 3));
     HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,

@@ -116,6 +116,16 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
                        int dual, const int32_t* nloc, const int32_t* act_p,
                        int a0, hipStream_t s);
 
+// Register-resident eigensolve (eigreg.hip) of the `cnt` slots in `list`,
+// every one of size <= nb (nb in {16, 32, 48, 64}).  Q may alias V.
+long long eigr_rot_capacity(int K);
+long long eigr_sweep_capacity(int K);
+hipError_t launch_eigr(int nb, int K, int dual, const int32_t* nloc,
+                       const int32_t* act_p, int a0, const int32_t* list,
+                       int cnt, const double* G, double* Q, double* D,
+                       double* E, double* cs, int32_t* hdr, int32_t* count,
+                       double* lam, double* V, int* status, hipStream_t s);
+
 // Weights + ensemble transform + RTPP/RTPS + round_off for points [p0,p1).
 template <typename T>
 hipError_t launch_transform(const KParams& kp, const Tables<T>& tb,
