@@ -185,7 +185,7 @@ def main():
     # --- roofline of the dominant kernel (stage times measured with HIP events) ---
     L = st["local_obs"]
     na = st["active_points"]
-    nch = -(-P // max(1024, min((1 << 31) // (8 * K * K), 65536)))
+    nch = int(st["chunks"])
     stages = {
         "gram": dict(ms=stage["ms_gram"], work=2.0 * L * K * K, unit="TFLOP/s", bound="mfma",
                      peak=PEAK_FP64_TFLOPS, kernel="k_gram_fp64"),

@@ -110,6 +110,8 @@ typedef struct letkf_stats {
   double ms_total, ms_search, ms_gram, ms_eig, ms_transform;
   double gram_flops;       /* 2*n*K^2 (or 2*n^2*K dual) summed          */
   double transform_bytes;  /* fcst read + anal written                 */
+  int64_t chunks;          /* point chunks (one launch of each per-chunk */
+                           /* kernel: fill, Gram, transform)            */
 } letkf_stats_t;
 
 typedef struct letkf_handle letkf_handle_t;

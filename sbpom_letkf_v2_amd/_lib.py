@@ -52,7 +52,7 @@ class Stats(C.Structure):
         "points", "active_points", "local_obs", "dual_points", "max_nobsl")] + [
         (n, C.c_double) for n in (
             "ms_total", "ms_search", "ms_gram", "ms_eig", "ms_transform",
-            "gram_flops", "transform_bytes")]
+            "gram_flops", "transform_bytes")] + [("chunks", C.c_int64)]
 
 
 class EngineError(RuntimeError):

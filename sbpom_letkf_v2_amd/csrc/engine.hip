@@ -289,8 +289,19 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
 
   // chunks of consecutive points sized so that the per-point K x K
   // workspaces (G, V) stay within ~2 GiB each
-  long long PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
-  PC = std::max(1024LL, std::min(PC, 65536LL));
+  // Chunk of consecutive points: its active slots own a K x K Gram and
+  // eigenvector block and the eigensolve's rotation log.  The register
+  // eigensolve (K <= 64) wants many matrices per launch, so its chunks are
+  // sized by a ~24 GB workspace budget; the LDS path keeps 2 GiB per block.
+  long long PC;
+  if (K <= 64) {
+    const double per_slot = 16.0 * K * K + 16.0 * eigr_rot_capacity(K) + 8.0 * eigr_sweep_capacity(K) + 64.0 * K;
+    PC = (long long)(24e9 / per_slot);
+  } else {
+    PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
+    PC = std::min(PC, 65536LL);
+  }
+  PC = std::max(1024LL, std::min(PC, P));
   const long long nch = (P + PC - 1) / PC;
   std::vector<long long> bpts(nch + 1);
   for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
@@ -352,8 +363,8 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   HIP_TRY(h->V.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
   HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
-  const long long cap = reg_eig ? 2 * eigr_rot_capacity(K) : eig_log_capacity(K);
-  const long long hcap = reg_eig ? 2 * eigr_sweep_capacity(K) : eig_log_capacity(K);
+  const long long cap = reg_eig ? 2 * eigr_rot_capacity(K) : eig_log_capacity(K);   // doubles (c,s)
+  const long long hcap = reg_eig ? 2 * eigr_sweep_capacity(K) : eig_log_capacity(K); // int32 (col,len)
   HIP_TRY(h->eD.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->eE.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->logc.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
@@ -427,6 +438,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
       }
     }
   }
+  st.chunks = nch;
   st.gram_flops = 2.0 * (double)st.local_obs * K * K;
   st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
   h->stats = st;
