@@ -208,15 +208,45 @@ __global__ void k_bucket_count(long long P, const int32_t* nloc, int K, int dual
   }
 }
 
-__global__ void k_bucket_scatter(long long P, const int32_t* nloc, const int32_t* actidx, int K, int dual,
-                                 long long PC, int32_t* cursor, int32_t* list) {
-  for (long long p = blockIdx.x * 256LL + threadIdx.x; p < P; p += 256LL * gridDim.x) {
-    const int n = nloc[p];
-    if (n == 0) continue;
-    const long long c = p / PC;
-    const int q = (slot_size(n, K, dual) - 1) / 16;
-    const int pos = atomicAdd(&cursor[c * kNQ + q], 1);
-    list[pos] = actidx[p] - actidx[c * PC];
+// Stable-within-block scatter of active points into per-(chunk, bucket)
+// lists: each block takes 256 consecutive points, ranks them per bucket with
+// wave ballots and reserves one range per bucket, so lists stay close to
+// point order (neighbouring points then share cache lines in the update).
+__global__ void __launch_bounds__(256) k_bucket_scatter(long long P, const int32_t* nloc, const int32_t* actidx,
+                                                        int K, int dual, long long PC, int32_t* cursor,
+                                                        int32_t* list) {
+  __shared__ int wcnt[4][kNQ];
+  __shared__ int base[kNQ];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (long long p0 = blockIdx.x * 256LL; p0 < P; p0 += 256LL * gridDim.x) {
+    const long long p = p0 + threadIdx.x;
+    int q = -1;
+    if (p < P) {
+      const int n = nloc[p];
+      if (n > 0) q = (slot_size(n, K, dual) - 1) / 16;
+    }
+    // all points of a 256-block lie in one chunk when PC is a multiple of 256
+    const long long c = p0 / PC;
+    int rank[kNQ];
+#pragma unroll
+    for (int b = 0; b < kNQ; ++b) {
+      const unsigned long long m = __ballot(q == b);
+      rank[b] = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+      if (lane == 0) wcnt[wv][b] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < kNQ) {
+      const int b = threadIdx.x;
+      const int tot = wcnt[0][b] + wcnt[1][b] + wcnt[2][b] + wcnt[3][b];
+      base[b] = tot ? atomicAdd(&cursor[c * kNQ + b], tot) : 0;
+    }
+    __syncthreads();
+    if (q >= 0) {
+      int pos = base[q] + rank[q];
+      for (int w = 0; w < wv; ++w) pos += wcnt[w][q];
+      list[pos] = actidx[p] - actidx[c * PC];
+    }
+    __syncthreads();
   }
 }
 
@@ -296,7 +326,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   long long PC;
   if (K <= 64) {
     const double per_slot = 16.0 * K * K + 16.0 * eigr_rot_capacity(K) + 8.0 * eigr_sweep_capacity(K) + 64.0 * K;
-    PC = (long long)(24e9 / per_slot);
+    PC = (long long)(24e9 / per_slot) / 256 * 256;  // multiple of 256 (k_bucket_scatter)
   } else {
     PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
     PC = std::min(PC, 65536LL);
@@ -386,6 +416,11 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     }
     HIP_TRY(mark(e0 + 1));
     if (na > 0) {
+      if (p.precision != LETKF_PREC_FP64 && K <= 64)
+        HIP_TRY(launch_gram_mixed<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
+                                     h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0,
+                                     na, h->G.as<double>(), h->bvec.as<double>(), dual, s));
+      else
       HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
                                   lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na, h->G.as<double>(),
                                   h->bvec.as<double>(), dual, s));
@@ -396,11 +431,13 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
         for (int q = 0; q < kNQ; ++q) {
           const int cnt = hist[(size_t)c * kNQ + q];
           if (cnt == 0) continue;
-          HIP_TRY(launch_eigr(16 * (q + 1), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
-                              h->blist.as<int32_t>() + boff[(size_t)c * kNQ + q], cnt, h->G.as<double>(),
-                              h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(), h->logc.as<double>(),
-                              h->logi.as<int32_t>(), h->logn.as<int32_t>(), h->lam.as<double>(),
-                              h->V.as<double>(), h->status.as<int>(), s));
+          HIP_TRY(launch_eigr_update<T>(16 * (q + 1), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
+                                        h->blist.as<int32_t>() + boff[(size_t)c * kNQ + q], cnt, h->G.as<double>(),
+                                        h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
+                                        h->logc.as<double>(), h->logi.as<int32_t>(), h->logn.as<int32_t>(),
+                                        h->lam.as<double>(), h->V.as<double>(), kp, tb, f3, f2, a3, a2,
+                                        h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
+                                        h->yT.as<double>(), h->bvec.as<double>(), h->status.as<int>(), s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -413,7 +450,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,
                                 h->lam.as<double>(), h->V.as<double>(), h->bvec.as<double>(), f3, f2, a3, a2, dual,
                                 h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
-                                h->yT.as<double>(), s));
+                                h->yT.as<double>(), reg_eig ? h->logn.as<int32_t>() : nullptr, s));
     HIP_TRY(mark(e0 + 4));
   }
   HIP_TRY(hipEventRecord(ev1, s));

@@ -235,4 +235,138 @@ template hipError_t launch_gram_fp64<double>(const KParams&, const Tables<double
                                              const int32_t*, const int64_t*, long long, const int32_t*,
                                              const double*, int, int, double*, double*, int, hipStream_t);
 
+// ------------------------------------------------------------------------
+// Mixed-precision Gram on MFMA (K <= 64): the K x K (or dual n x n) product
+// runs as four 32x32 output tiles, one per wavefront, with FP32
+// accumulation:
+//   prec FP32   : v_mfma_f32_32x32x2f32 on FP32-rounded operands;
+//   prec BF16X3 : operands split x = hi + lo (bf16 each), three products
+//                 hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_bf16.
+// The innovation term b (and dt in the dual form) stays FP64; the result is
+// widened to FP64 for the eigensolve.  Lane layouts (checked on gfx950):
+//   32x32x2 f32 : a = A[i][k], i = lane%32, k = lane/32;  b = B[k][j], j = lane%32
+//   32x32x16 bf16: a = A[i][8*(lane/32)+q], b = B[8*(lane/32)+q][j], q = 0..7
+//   C (16 floats): C[(r/4)*8 + (lane/32)*4 + r%4][lane%32]
+typedef float mf_v16 __attribute__((ext_vector_type(16)));
+typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int kMfPanel = 32;  // obs (primal) or members (dual) per LDS panel
+
+template <int PREC, typename Src>
+__global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, double* __restrict__ G,
+                                                   double* __restrict__ bvec, int dual) {
+  // panel: rows = summation index (obs or member), cols = output index (64)
+  __shared__ float As[kMfPanel][64 + 4];
+  __shared__ float Bs[kMfPanel][64 + 4];
+  __shared__ double sws[64];
+  __shared__ int ios[64];
+  const int n = src.begin(blockIdx.x);
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int r0 = (wv >> 1) * 32, c0 = (wv & 1) * 32;  // output tile of this wave
+  const bool dm = dual && n < K;
+  const int N = dm ? n : K;                         // output dimension
+  const int S = dm ? K : n;                         // summation length
+  mf_v16 acc = {};
+  if (dm) {
+    for (int l = t; l < n; l += 256) {
+      int io;
+      double sw;
+      src.brow(l, io, sw);
+      ios[l] = io;
+      sws[l] = sw;
+      bvec[(long long)blockIdx.x * K + l] = sw * src.dep(l);
+    }
+  } else if (t < K) {
+    // b = Rlinv_dYf^T dep in FP64 (member t)
+    double bacc = 0.0;
+    for (int l = 0; l < n; ++l) {
+      double y, rl, rd;
+      src.row(l, K, t, y, rl, rd);
+      bacc = fma(y * rl / rd, src.dep(l), bacc);
+    }
+    bvec[(long long)blockIdx.x * K + t] = bacc;
+  }
+  __syncthreads();
+  for (int s0 = 0; s0 < S; s0 += kMfPanel) {
+    const int ps = min(kMfPanel, S - s0);
+    for (int q = t; q < kMfPanel * 64; q += 256) {
+      const int row = q >> 6, col = q & 63;
+      double av = 0.0, bv = 0.0;
+      if (row < ps && col < N) {
+        if (dm) {
+          double y;
+          src.row_of(ios[col], K, s0 + row, y);
+          av = bv = sws[col] * y;
+        } else {
+          double rl, rd;
+          src.row(s0 + row, K, col, bv, rl, rd);
+          av = bv * rl / rd;
+        }
+      }
+      // FP32 panel; BF16X3 splits it into hi/lo bf16 at use (16 bits kept)
+      As[row][col] = (float)av;
+      Bs[row][col] = (float)bv;
+    }
+    __syncthreads();
+    if (PREC == LETKF_PREC_FP32) {
+      for (int k = 0; k < ps; k += 2) {
+        const int kk = k + (lane >> 5);
+        const float a = kk < ps ? As[kk][r0 + (lane & 31)] : 0.f;
+        const float b = kk < ps ? Bs[kk][c0 + (lane & 31)] : 0.f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+      }
+    } else {
+      for (int k = 0; k < ps; k += 16) {
+        mf_bf8 ah, al, bh, bl;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int kk = k + 8 * (lane >> 5) + q;
+          const float a = kk < ps ? As[kk][r0 + (lane & 31)] : 0.f;
+          const float b = kk < ps ? Bs[kk][c0 + (lane & 31)] : 0.f;
+          const __bf16 a1 = (__bf16)a, b1 = (__bf16)b;
+          ah[q] = a1;
+          bh[q] = b1;
+          al[q] = (__bf16)(a - (float)a1);
+          bl[q] = (__bf16)(b - (float)b1);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  double* Gp = G + (long long)blockIdx.x * K * K;
+  const int col = c0 + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = r0 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+    if (row < N && col < N) Gp[row * N + col] = (double)acc[r] + (!dm && row == col ? diag : 0.0);
+  }
+}
+
+template <typename T>
+hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, const double* yT,
+                             const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
+                             const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
+                             hipStream_t s) {
+  if (na <= 0) return hipSuccess;
+  if (kp.K > 64) return hipErrorInvalidValue;
+  ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
+  const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
+  if (prec == LETKF_PREC_FP32)
+    hipLaunchKernelGGL((k_gram_mfma<LETKF_PREC_FP32, ObsSource<T>>), dim3(na), dim3(256), 0, s, kp.K, diag, src,
+                       G, bvec, dual);
+  else
+    hipLaunchKernelGGL((k_gram_mfma<LETKF_PREC_BF16X3, ObsSource<T>>), dim3(na), dim3(256), 0, s, kp.K, diag,
+                       src, G, bvec, dual);
+  return hipGetLastError();
+}
+template hipError_t launch_gram_mixed<float>(int, const KParams&, const Tables<float>&, const double*,
+                                             const int32_t*, const int64_t*, long long, const int32_t*,
+                                             const double*, int, int, double*, double*, int, hipStream_t);
+template hipError_t launch_gram_mixed<double>(int, const KParams&, const Tables<double>&, const double*,
+                                              const int32_t*, const int64_t*, long long, const int32_t*,
+                                              const double*, int, int, double*, double*, int, hipStream_t);
+
 }  // namespace letkf

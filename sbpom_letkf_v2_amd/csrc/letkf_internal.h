@@ -71,6 +71,17 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
                             int na, double* G, double* bvec, int dual,
                             hipStream_t s);
 
+// The same products with the Gram on MFMA in FP32 (prec LETKF_PREC_FP32)
+// or from BF16 hi/lo splits (LETKF_PREC_BF16X3), FP32 accumulation, b in
+// FP64; K <= 64.
+template <typename T>
+hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb,
+                             const double* yT, const int32_t* act_p,
+                             const int64_t* loff, long long lbase,
+                             const int32_t* lobs, const double* lrloc, int a0,
+                             int na, double* G, double* bvec, int dual,
+                             hipStream_t s);
+
 // Same products for letkf_core_batched: problem b owns rows offs[b]..offs[b+1)
 // of hdxf (K values per row); nrow[b] receives the row count.
 template <typename T>
@@ -126,6 +137,22 @@ hipError_t launch_eigr(int nb, int K, int dual, const int32_t* nloc,
                        double* E, double* cs, int32_t* hdr, int32_t* count,
                        double* lam, double* V, int* status, hipStream_t s);
 
+// Register eigensolve fused with the ensemble update (K <= 64): writes the
+// analysis of every listed slot's point; slots whose rotation log overflowed
+// get lam/V instead (count[slot] == -1) for launch_transform's fallback mode.
+template <typename T>
+hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
+                              const int32_t* act_p, int a0, const int32_t* list,
+                              int cnt, const double* G, double* Q, double* D,
+                              double* E, double* cs, int32_t* hdr,
+                              int32_t* count, double* lam, double* V,
+                              const KParams& kp, const Tables<T>& tb,
+                              const T* X3, const T* X2, T* A3, T* A2,
+                              const int64_t* loff, long long lbase,
+                              const int32_t* lobs, const double* lrloc,
+                              const double* yT, const double* bvec,
+                              int* status, hipStream_t s);
+
 // Weights + ensemble transform + RTPP/RTPS + round_off for points [p0,p1).
 template <typename T>
 hipError_t launch_transform(const KParams& kp, const Tables<T>& tb,
@@ -136,7 +163,7 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb,
                             T* anal2d, int dual, const int64_t* loff,
                             long long lbase, const int32_t* lobs,
                             const double* lrloc, const double* yT,
-                            hipStream_t s);
+                            const int32_t* fallback_count, hipStream_t s);
 
 template <typename T>
 hipError_t launch_copy(const T* src, T* dst, size_t n, hipStream_t s);

@@ -40,6 +40,9 @@ constexpr int kMaxVars = 8;
 // FP64 copy of obshdxf.  enabled == 0 for the plain form.
 struct DualRows {
   int enabled;
+  // non-null: update only points without local obs and points whose slot
+  // count is -1 (the others were updated by the fused eigensolve)
+  const int32_t* fallback_count;
   const int64_t* loff;
   long long lbase;
   const int32_t* lobs;
@@ -100,6 +103,7 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
     const bool surf = p >= surf0;
     const int nvp = surf ? nvt : nv3;
     const int n = nloc[p];
+    if (dr.fallback_count && n > 0 && dr.fallback_count[actidx[p] - a0] != -1) continue;
     // per-variable forecast mean and perturbations (members m0, m1)
     double dx0[kMaxVars], dx1[kMaxVars], fm[kMaxVars], fs[kMaxVars], ta0[kMaxVars], ta1[kMaxVars];
     _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
@@ -335,6 +339,7 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
     const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
     if (tp >= ntile) continue;
     const long long p = pt + tp;
+    if (dr.fallback_count && nloc[p] > 0 && dr.fallback_count[actidx[p] - a0] != -1) continue;
     if (v < nv3)
       A3[p + P * (m + (long long)K * v)] = (T)xs[e];
     else if (p >= surf0)
@@ -347,14 +352,15 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0
                             const int32_t* nloc, const int32_t* actidx, int a0, const double* lam,
                             const double* V, const double* bvec, const T* fcst3d, const T* fcst2d, T* anal3d,
                             T* anal2d, int dual, const int64_t* loff, long long lbase, const int32_t* lobs,
-                            const double* lrloc, const double* yT, hipStream_t s) {
+                            const double* lrloc, const double* yT, const int32_t* fallback_count,
+                            hipStream_t s) {
   if (p1 <= p0) return hipSuccess;
   const int nvt = kp.nv3d + kp.nv2d;
   if (nvt > kMaxVars) return hipErrorInvalidValue;
   const int TI = kp.K <= 64 ? 16 : 8;
   const size_t lds = sizeof(double) * ((size_t)nvt * kp.K * TI +
                                        4 * (2 * (size_t)(1 + nvt) * kp.K + kp.K / 2 + 1));
-  DualRows dr{dual, loff, lbase, lobs, lrloc, yT, (double)((T)(kp.K - 1) / (T)kp.rho)};
+  DualRows dr{dual, fallback_count, loff, lbase, lobs, lrloc, yT, (double)((T)(kp.K - 1) / (T)kp.rho)};
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_transform<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -370,11 +376,11 @@ template hipError_t launch_transform<float>(const KParams&, const Tables<float>&
                                             const int32_t*, const int32_t*, int, const double*, const double*,
                                             const double*, const float*, const float*, float*, float*, int,
                                             const int64_t*, long long, const int32_t*, const double*,
-                                            const double*, hipStream_t);
+                                            const double*, const int32_t*, hipStream_t);
 template hipError_t launch_transform<double>(const KParams&, const Tables<double>&, long long, long long,
                                              const int32_t*, const int32_t*, int, const double*, const double*,
                                              const double*, const double*, const double*, double*, double*,
                                              int, const int64_t*, long long, const int32_t*, const double*,
-                                             const double*, hipStream_t);
+                                             const double*, const int32_t*, hipStream_t);
 
 }  // namespace letkf

@@ -216,3 +216,17 @@ def test_aoei_gpu_vs_oracle():
     _lib.check(_lib.lib().letkf_aoei(eng.h, 1000, vp(te), vp(td), vp(ts)))
     eng.close()
     assert np.array_equal(te.cpu().numpy(), po.aoei(e, d, s))
+
+
+@pytest.mark.parametrize("precision,form", [("fp32", "batched"), ("bf16x3", "batched"), ("fp32", "plain")])
+def test_mixed_precision_gram_close_to_fp64(precision, form):
+    """Gram on MFMA (FP32, or BF16 hi/lo split) with FP64 eigensolve: the
+    analysis must stay within 1e-4 of the FP64 oracle relative to the field
+    scale (FP32 products keep ~7 digits; the innovation term stays FP64)."""
+    _say(f"case {precision}/{form}")
+    pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=48, nobs=900, seed=61, alpha_rtpp=0.7,
+                            real_kind=8, sigma_h_cells=1.0, sigma_v=1.0)
+    a3, a2, a3o, a2o, nl, st = _run_both(pr, precision=precision, form=form)
+    e3 = per_var_err(a3, a3o, axis_var=3)
+    _say(f"  err3={e3}")
+    assert max(e3) <= 1e-4
