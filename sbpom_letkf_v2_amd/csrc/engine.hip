@@ -325,7 +325,9 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   // sized by a ~24 GB workspace budget; the LDS path keeps 2 GiB per block.
   long long PC;
   if (K <= 64) {
-    const double per_slot = 16.0 * K * K + 16.0 * eigr_rot_capacity(K) + 8.0 * eigr_sweep_capacity(K) + 64.0 * K;
+    const double ncol = 1.0 + p.nv3d + p.nv2d;
+    const double per_slot = 16.0 * K * K + 8.0 * K * ncol + 16.0 * eigr_rot_capacity(K) +
+                            (double)eigr_ix_pitch(K) + 8.0 * eigr_sweep_capacity(K) + 64.0 * K + 8.0;
     PC = (long long)(24e9 / per_slot) / 256 * 256;  // multiple of 256 (k_bucket_scatter)
   } else {
     PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
@@ -391,6 +393,9 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   HIP_TRY(h->lrloc.ensure(sizeof(double) * (maxL + 1)));
   HIP_TRY(h->G.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
   HIP_TRY(h->V.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
+  // vector blocks of the register path (rows x (1 + nv3d + nv2d) per slot)
+  const long long yp = (long long)K * (1 + p.nv3d + p.nv2d);
+  if (reg_eig) HIP_TRY(h->yvec.ensure(sizeof(double) * (size_t)(maxA + 1) * yp));
   HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   const long long cap = reg_eig ? 2 * eigr_rot_capacity(K) : eig_log_capacity(K);   // doubles (c,s)
@@ -401,6 +406,10 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   if (!reg_eig) HIP_TRY(h->logs.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
   HIP_TRY(h->logi.ensure(sizeof(int32_t) * (size_t)(maxA + 1) * hcap));
   HIP_TRY(h->logn.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
+  if (reg_eig) {
+    HIP_TRY(h->logx.ensure((size_t)(maxA + 1) * eigr_ix_pitch(K)));
+    HIP_TRY(h->logr.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
+  }
 
   if (timing) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
   for (long long c = 0; c < nch; ++c) {
@@ -434,8 +443,9 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
           HIP_TRY(launch_eigr_update<T>(16 * (q + 1), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
                                         h->blist.as<int32_t>() + boff[(size_t)c * kNQ + q], cnt, h->G.as<double>(),
                                         h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
-                                        h->logc.as<double>(), h->logi.as<int32_t>(), h->logn.as<int32_t>(),
-                                        h->lam.as<double>(), h->V.as<double>(), kp, tb, f3, f2, a3, a2,
+                                        h->logc.as<double>(), h->logx.as<uint8_t>(), h->logi.as<int32_t>(),
+                                        h->logn.as<int32_t>(), h->logr.as<int32_t>(), h->lam.as<double>(),
+                                        h->V.as<double>(), h->yvec.as<double>(), yp, kp, tb, f3, f2, a3, a2,
                                         h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
                                         h->yT.as<double>(), h->bvec.as<double>(), h->status.as<int>(), s));
         }

@@ -127,25 +127,23 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
                        int dual, const int32_t* nloc, const int32_t* act_p,
                        int a0, hipStream_t s);
 
-// Register-resident eigensolve (eigreg.hip) of the `cnt` slots in `list`,
-// every one of size <= nb (nb in {16, 32, 48, 64}).  Q may alias V.
+// Register-resident eigensolve (eigreg.inc, K <= 64) fused with the
+// ensemble update: writes the analysis of the point of every listed slot
+// (all of size <= nb, nb in {16, 32, 48, 64}).  Per-slot workspaces: Q K*K,
+// D/E K, cs 2*eigr_rot_capacity, ix eigr_ix_pitch bytes, hdr
+// 2*eigr_sweep_capacity, count and nrot 1, Y yp >= K*(1+nv3d+nv2d); Q may
+// alias V.  Slots whose rotation log overflowed get lam/V instead
+// (count[slot] == -1) for launch_transform's fallback mode.
 long long eigr_rot_capacity(int K);
 long long eigr_sweep_capacity(int K);
-hipError_t launch_eigr(int nb, int K, int dual, const int32_t* nloc,
-                       const int32_t* act_p, int a0, const int32_t* list,
-                       int cnt, const double* G, double* Q, double* D,
-                       double* E, double* cs, int32_t* hdr, int32_t* count,
-                       double* lam, double* V, int* status, hipStream_t s);
-
-// Register eigensolve fused with the ensemble update (K <= 64): writes the
-// analysis of every listed slot's point; slots whose rotation log overflowed
-// get lam/V instead (count[slot] == -1) for launch_transform's fallback mode.
+long long eigr_ix_pitch(int K);
 template <typename T>
 hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const int32_t* act_p, int a0, const int32_t* list,
                               int cnt, const double* G, double* Q, double* D,
-                              double* E, double* cs, int32_t* hdr,
-                              int32_t* count, double* lam, double* V,
+                              double* E, double* cs, uint8_t* ix, int32_t* hdr,
+                              int32_t* count, int32_t* nrot, double* lam,
+                              double* V, double* Y, long long yp,
                               const KParams& kp, const Tables<T>& tb,
                               const T* X3, const T* X2, T* A3, T* A2,
                               const int64_t* loff, long long lbase,
