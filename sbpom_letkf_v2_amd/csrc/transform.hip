@@ -72,11 +72,24 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
   double* z_s = q_s + K;                             // bank A: q[K], z[nvt][K]
   double* y_s = q_s + (size_t)(1 + nvt) * K;         // bank B: [nvt+1][K]
   int* io_s = (int*)(y_s + (size_t)(1 + nvt) * K);   // K obs indices
+  // in fallback mode a tile with no point left to this kernel (no local obs,
+  // or an overflowed eigensolve log) is done before anything is staged
+  // (and only the points left to it are staged)
+  __shared__ int need_s[16];
+  if (t < TI) {
+    bool need = t < ntile;
+    if (need && dr.fallback_count) {
+      const long long p = pt + t;
+      need = nloc[p] == 0 || dr.fallback_count[actidx[p] - a0] == -1;
+    }
+    need_s[t] = need;
+  }
+  if (!__syncthreads_or(t < TI && need_s[t])) return;
   // ---- stage the forecast tile ----
   for (int e = t; e < nvt * K * TI; e += 256) {
     const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
     double x = 0.0;
-    if (tp < ntile) {
+    if (need_s[tp]) {
       const long long p = pt + tp;
       if (v < nv3)
         x = (double)X3[p + P * (m + (long long)K * v)];
