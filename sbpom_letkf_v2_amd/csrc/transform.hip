@@ -85,7 +85,8 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
     need_s[t] = need;
   }
   if (!__syncthreads_or(t < TI && need_s[t])) return;
-  // ---- stage the forecast tile ----
+  // ---- stage the forecast tile (unrolled: several loads in flight) ----
+#pragma unroll 8
   for (int e = t; e < nvt * K * TI; e += 256) {
     const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
     double x = 0.0;
