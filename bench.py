@@ -8,7 +8,9 @@ point, inputs resident in HBM.
 
 Forms:
   plain    FP64 throughout, K x K eigensolve at every point with local obs
-  batched  the chosen form (see --form); until it exists this is plain
+  batched  points with nobsl < K solved in the nobsl x nobsl form; with
+           --precision fp32 / bf16x3 the Gram products run on MFMA (FP64
+           eigensolve).  Default: batched + fp32, timed next to plain FP64.
 
 Prints one JSON line (rank 0).  Progress goes to stderr.
 """
@@ -23,7 +25,9 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 
 METRIC = "analysed grid pts/s at K=64: plain FP64 vs batched mixed-prec; max err vs FP64"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}   # dense, per operand type of the Gram
 PEAK_HBM_GBS = 8000.0
+EIG_KINDS = ["k_eigr_tridiag", "k_eigr_ql", "k_eigr_qvec", "k_eigr_vrot", "k_eigr_vout"]
 
 
 def log(*a):
@@ -36,8 +40,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="fp64")
-    ap.add_argument("--form", default="plain")
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--form", default="batched", choices=["plain", "batched"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -49,7 +53,7 @@ def run_form(prd, precision, form, steps, warmup):
     import torch
     from sbpom_letkf_v2_amd import engine
     prm = engine.make_params(prd, precision, form)
-    prm.stage_timing = 1
+    prm.stage_timing = 2        # stage marks + every eigensolve kernel launch
     eng = engine.Engine(prm)
     a3 = torch.empty_like(prd.fcst3d)
     a2 = torch.empty_like(prd.fcst2d)
@@ -59,18 +63,28 @@ def run_form(prd, precision, form, steps, warmup):
         log(f"  warmup {w}: {time.perf_counter() - t:.2f}s {eng.stats()}")
     torch.cuda.synchronize()
     tot = {k: 0.0 for k in ("ms_search", "ms_gram", "ms_eig", "ms_transform", "ms_total")}
+    kms = [0.0] * 20
+    kn = [0] * 20
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):
         eng.da_main_device(prd, a3, a2)
         st = eng.stats()
         for k in tot:
             tot[k] += st[k]
+        for q in range(20):
+            kms[q] += st["ms_eigk"][q]
+            kn[q] += st["n_eigk"][q]
         log(f"  step {s}: {st['ms_total']:.1f} ms")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st = eng.stats()
     eng.close()
-    return a3, a2, wall / steps, {k: v / steps for k, v in tot.items()}, st
+    stage = {k: v / steps for k, v in tot.items()}
+    stage["eig_kernels"] = {EIG_KINDS[q // 4] + f"<{16 * (q % 4 + 1)}>": {"ms": kms[q] / steps,
+                                                                          "launches": kn[q] / steps}
+                            for q in range(20) if kn[q]}
+    return a3, a2, wall / steps, stage, st
 
 
 def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2):
@@ -176,38 +190,53 @@ def main():
     if (args.precision, args.form) != ("fp64", "plain") and not args.no_plain:
         log("timing plain FP64 form")
         ref3, ref2, psec, pstage, pst = run_form(prd, "fp64", "plain", max(1, args.steps // 2), 1)
-        result_extra["plain_fp64"] = {"value": P / psec, "ms_per_step": psec * 1e3, "stage_ms": pstage}
+        result_extra["plain_fp64"] = {"value": P / psec, "ms_per_step": psec * 1e3,
+                                      "stage_ms": {k: v for k, v in pstage.items() if k.startswith("ms_")}}
         result_extra["speedup_vs_plain_fp64"] = psec / sec
         result_extra["errors_vs_fp64"] = field_errors(a3, a2, ref3, ref2, nv3, nv2, K)
         log(f"  speedup {psec / sec:.2f}x; errors {result_extra['errors_vs_fp64']['summary']}")
     elif args.precision == "fp64":
         ref3, ref2 = a3, a2
-    # --- roofline of the dominant kernel (stage times measured with HIP events) ---
+    # --- roofline of the dominant kernel: HIP events around its launches in the timed steps ---
     L = st["local_obs"]
     na = st["active_points"]
     nch = int(st["chunks"])
-    stages = {
-        "gram": dict(ms=stage["ms_gram"], work=2.0 * L * K * K, unit="TFLOP/s", bound="mfma",
-                     peak=PEAK_FP64_TFLOPS, kernel="k_gram_fp64"),
-        "eig": dict(ms=stage["ms_eig"], work=9.0 * na * K ** 3, unit="TFLOP/s", bound="mfma",
-                    peak=PEAK_FP64_TFLOPS, kernel="k_eig_fp64"),
-        "transform": dict(ms=stage["ms_transform"], work=2.0 * 8 * K * (P * nv3 + nij1 * nv2),
-                          unit="GB/s", bound="hbm", peak=PEAK_HBM_GBS, kernel="k_transform"),
+    ek = stage["eig_kernels"]
+
+    def family(kind):
+        ms = sum(v["ms"] for k, v in ek.items() if k.startswith(kind + "<"))
+        n = sum(v["launches"] for k, v in ek.items() if k.startswith(kind + "<"))
+        return ms, n
+
+    tri_ms, tri_n = family("k_eigr_tridiag")
+    gram_kernel = "k_gram_fp64" if args.precision == "fp64" or K > 64 else "k_gram_mfma"
+    kernels = {
+        # Gram: 2 n K^2 (primal) or 2 n^2 K (dual) flops per point, one launch per chunk
+        gram_kernel: dict(ms=stage["ms_gram"], launches=nch, work=st["gram_flops"], unit="TFLOP/s",
+                          bound="mfma", peak=PEAK_TFLOPS[args.precision]),
+        # Householder tridiagonalisation: (4/3) m^3 flops per point, FP64
+        "k_eigr_tridiag": dict(ms=tri_ms, launches=tri_n, work=st["eig_flops"], unit="TFLOP/s",
+                               bound="mfma", peak=PEAK_FP64_TFLOPS),
+        # tile update (points without local obs and overflowed eigensolves): read + write of the ensemble
+        "k_transform": dict(ms=stage["ms_transform"], launches=nch, work=st["transform_bytes"],
+                            unit="GB/s", bound="hbm", peak=PEAK_HBM_GBS),
     }
-    dom_name = max(stages, key=lambda k: stages[k]["ms"])
-    dom = stages[dom_name]
+    kernels = {k: v for k, v in kernels.items() if v["launches"] and v["ms"] > 0}
+    dom_name = max(kernels, key=lambda k: kernels[k]["ms"])
+    dom = kernels[dom_name]
     scale = 1e12 if dom["unit"] == "TFLOP/s" else 1e9
     achieved = dom["work"] / (dom["ms"] * 1e-3) / scale
-    per_launch_ms = dom["ms"] / nch
     roofline = {"bound": dom["bound"], "achieved": achieved, "peak": dom["peak"], "unit": dom["unit"],
-                "frac": achieved / dom["peak"], "traffic": None, "kernel": dom["kernel"],
-                "launches_per_step": nch, "avg_launch_ms": per_launch_ms,
-                "work_per_launch": dom["work"] / nch}
-    stage_roofline = {}
-    for k, v in stages.items():
+                "frac": achieved / dom["peak"], "traffic": None, "kernel": dom_name,
+                "launches_per_step": dom["launches"], "avg_launch_ms": dom["ms"] / dom["launches"],
+                "work_per_launch": dom["work"] / dom["launches"]}
+    kernel_roofline = {}
+    for k, v in kernels.items():
         sc = 1e12 if v["unit"] == "TFLOP/s" else 1e9
         a = v["work"] / max(v["ms"] * 1e-3, 1e-12) / sc
-        stage_roofline[k] = {"ms": v["ms"], "achieved": a, "unit": v["unit"], "frac": a / v["peak"]}
+        kernel_roofline[k] = {"ms": v["ms"], "launches": v["launches"], "achieved": a, "unit": v["unit"],
+                              "frac": a / v["peak"]}
+    stage_ms = {k: v for k, v in stage.items() if k.startswith("ms_")}
     # --- CPU baseline + full-size parity at the sampled points ---
     cpu = None
     parity = None
@@ -235,7 +264,8 @@ def main():
                                f"({meta['obs_layout']}), {args.precision}/{args.form}",
                    "points": P, "active_points": na, "local_obs": L,
                    "precision": args.precision, "form": args.form},
-        "roofline": roofline, "stage_roofline": stage_roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "kernel_roofline": kernel_roofline, "stage_ms": stage_ms,
+        "eig_kernel_ms": {k: v["ms"] for k, v in ek.items()}, "cpu_baseline": cpu,
         "parity_max_rel_err_vs_fp64_oracle_sample": parity,
     }
     out.update(result_extra)

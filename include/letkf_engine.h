@@ -71,7 +71,8 @@ typedef struct letkf_params {
   double dist_zero, dist_zerov; /* cut-offs, r_size values   (setting:97-98) */
   double tmin, tmax, smin, smax;/* round_off bounds          (setting:130-131) */
   int32_t stage_timing;         /* 1: time each stage with HIP events into */
-                                /* letkf_stats_t (adds event records only)  */
+                                /* letkf_stats_t (adds event records only); */
+                                /* 2: also each eigensolve kernel launch    */
 } letkf_params_t;
 
 /* Local grid of the analysis (common_setting.f90:168-176). */
@@ -112,6 +113,14 @@ typedef struct letkf_stats {
   double transform_bytes;  /* fcst read + anal written                 */
   int64_t chunks;          /* point chunks (one launch of each per-chunk */
                            /* kernel: fill, Gram, transform)            */
+  double eig_flops;        /* (4/3) m^3 summed over active points, m the */
+                           /* eigenproblem size (Householder reduction) */
+  double ms_eigk[20];      /* stage_timing 2, K <= 64: summed time of the */
+  int64_t n_eigk[20];      /* eigensolve kernels and their launch counts, */
+                           /* index 4*kind + bucket; kind 0 tridiagonal- */
+                           /* isation, 1 QL, 2 Q^T on the vectors, 3     */
+                           /* rotation replay, 4 Q and the update;       */
+                           /* bucket 0..3 = size <= 16, 32, 48, 64       */
 } letkf_stats_t;
 
 typedef struct letkf_handle letkf_handle_t;

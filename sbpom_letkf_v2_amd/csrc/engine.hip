@@ -165,31 +165,36 @@ __global__ void k_gather_bounds(int nb, const long long* pts, const int64_t* lof
   }
 }
 
-// max nobsl and the number of points with 0 < nobsl < K (solved in the
-// nobsl x nobsl form by the batched form).
-__global__ void k_point_stats(long long P, const int32_t* nloc, int K, unsigned long long* out) {
-  __shared__ int smax[256];
-  __shared__ unsigned long long sdual[256];
-  int mx = 0;
-  unsigned long long du = 0;
+// Per-call point statistics: out[0] max nobsl, out[1] dual-form points,
+// out[2] sum of n^2 over dual points, out[3] sum of n over the others
+// (Gram flops 2 n^2 K and 2 n K^2), out[4] sum of m^3 over active points,
+// m the eigenproblem size.
+__global__ void k_point_stats(long long P, const int32_t* nloc, int K, int dual, unsigned long long* out) {
+  __shared__ unsigned long long red[5][256];
+  unsigned long long a[5] = {0, 0, 0, 0, 0};
   for (long long p = blockIdx.x * 256LL + threadIdx.x; p < P; p += 256LL * gridDim.x) {
-    const int n = nloc[p];
-    mx = n > mx ? n : mx;
-    du += (n > 0 && n < K);
+    const unsigned long long n = (unsigned long long)nloc[p];
+    a[0] = n > a[0] ? n : a[0];
+    if (n == 0) continue;
+    const bool d = dual && n < (unsigned long long)K;
+    a[1] += d;
+    a[2] += d ? n * n : 0;
+    a[3] += d ? 0 : n;
+    const unsigned long long m = d ? n : (unsigned long long)K;
+    a[4] += m * m * m;
   }
-  smax[threadIdx.x] = mx;
-  sdual[threadIdx.x] = du;
+  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = a[q];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
-      smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
-      sdual[threadIdx.x] += sdual[threadIdx.x + o];
+      red[0][threadIdx.x] = max(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+      for (int q = 1; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomicMax(out, (unsigned long long)smax[0]);
-    atomicAdd(out + 1, sdual[0]);
+    atomicMax(out, red[0][0]);
+    for (int q = 1; q < 5; ++q) atomicAdd(out + q, red[q][0]);
   }
 }
 
@@ -337,19 +342,19 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   const long long nch = (P + PC - 1) / PC;
   std::vector<long long> bpts(nch + 1);
   for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
-  HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 2)));
+  HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 5)));
   long long* d_pts = h->bounds.as<long long>();
   long long* d_out = d_pts + (nch + 1);
   unsigned long long* d_pst = (unsigned long long*)(d_out + 2 * (nch + 1));
-  HIP_TRY(hipMemsetAsync(d_pst, 0, 2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_point_stats, dim3(1024), dim3(256), 0, s, P, h->nloc.as<int32_t>(), K, d_pst);
+  HIP_TRY(hipMemsetAsync(d_pst, 0, 5 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_point_stats, dim3(1024), dim3(256), 0, s, P, h->nloc.as<int32_t>(), K, dual, d_pst);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(d_pts, bpts.data(), sizeof(long long) * (nch + 1), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_gather_bounds, dim3((unsigned)((nch + 1 + 255) / 256)), dim3(256), 0, s, (int)(nch + 1),
                      d_pts, h->loff.as<int64_t>(), h->actidx.as<int32_t>(), d_out);
   HIP_TRY(hipGetLastError());
-  std::vector<long long> bv(2 * (nch + 1) + 2);
-  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * (2 * (nch + 1) + 2), hipMemcpyDeviceToHost, s));
+  std::vector<long long> bv(2 * (nch + 1) + 5);
+  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * (2 * (nch + 1) + 5), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   st.max_nobsl = bv[2 * (nch + 1)];
   st.dual_points = p.form == LETKF_FORM_BATCHED ? bv[2 * (nch + 1) + 1] : 0;
@@ -412,6 +417,14 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   }
 
   if (timing) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
+  // per-kernel marks of the register eigensolve: 5 kernels per bucket launch
+  EigTimer ktm{nullptr, nullptr, 0, 0};
+  if (p.stage_timing >= 2 && reg_eig) {
+    const int cap = (int)(5 * kNQ * nch);
+    HIP_TRY(h->ensure_kevents(2 * (size_t)cap));
+    h->ktags.assign(cap, 0);
+    ktm = EigTimer{h->kevents.data(), h->ktags.data(), cap, 0};
+  }
   for (long long c = 0; c < nch; ++c) {
     const size_t e0 = 2 + 5 * (size_t)c;
     const long long p0 = bpts[c], p1 = bpts[c + 1];
@@ -447,7 +460,8 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
                                         h->logn.as<int32_t>(), h->logr.as<int32_t>(), h->lam.as<double>(),
                                         h->V.as<double>(), h->yvec.as<double>(), yp, kp, tb, f3, f2, a3, a2,
                                         h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
-                                        h->yT.as<double>(), h->bvec.as<double>(), h->status.as<int>(), s));
+                                        h->yT.as<double>(), h->bvec.as<double>(), h->status.as<int>(),
+                                        ktm.ev ? &ktm : nullptr, s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -485,8 +499,15 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
       }
     }
   }
+  for (int u = 0; u < ktm.used; ++u) {
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, ktm.ev[2 * u], ktm.ev[2 * u + 1]));
+    st.ms_eigk[ktm.tag[u]] += t;
+    st.n_eigk[ktm.tag[u]] += 1;
+  }
   st.chunks = nch;
-  st.gram_flops = 2.0 * (double)st.local_obs * K * K;
+  st.gram_flops = 2.0 * K * ((double)bv[2 * (nch + 1) + 2] + (double)K * (double)bv[2 * (nch + 1) + 3]);
+  st.eig_flops = 4.0 / 3.0 * (double)bv[2 * (nch + 1) + 4];
   st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
   h->stats = st;
   return status_to_rc(status);

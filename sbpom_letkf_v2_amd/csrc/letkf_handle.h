@@ -58,6 +58,17 @@ struct letkf_handle {
   // staging for the host-pointer entry point
   DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
   letkf_stats_t stats{};
+  std::vector<hipEvent_t> kevents;  // eigensolve kernel marks (stage_timing 2)
+  std::vector<int> ktags;
+  hipError_t ensure_kevents(size_t n) {
+    while (kevents.size() < n) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      kevents.push_back(e);
+    }
+    return hipSuccess;
+  }
   std::vector<hipEvent_t> events;  // stage-timing marks, grown on demand
   hipError_t ensure_events(size_t n) {
     while (events.size() < n) {
@@ -74,6 +85,8 @@ struct letkf_handle {
     for (DevBuf* b : all) b->release();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();
+    for (hipEvent_t e : kevents) (void)hipEventDestroy(e);
+    kevents.clear();
   }
 };
 

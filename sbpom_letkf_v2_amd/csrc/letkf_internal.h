@@ -134,6 +134,14 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
 // 2*eigr_sweep_capacity, count and nrot 1, Y yp >= K*(1+nv3d+nv2d); Q may
 // alias V.  Slots whose rotation log overflowed get lam/V instead
 // (count[slot] == -1) for launch_transform's fallback mode.
+// Events around every eigensolve kernel launch (stage_timing 2): launch u
+// is bracketed by ev[2u], ev[2u+1] and tagged 4*kind + bucket.
+struct EigTimer {
+  hipEvent_t* ev;
+  int* tag;
+  int cap, used;
+};
+
 long long eigr_rot_capacity(int K);
 long long eigr_sweep_capacity(int K);
 long long eigr_ix_pitch(int K);
@@ -149,7 +157,7 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const int64_t* loff, long long lbase,
                               const int32_t* lobs, const double* lrloc,
                               const double* yT, const double* bvec,
-                              int* status, hipStream_t s);
+                              int* status, EigTimer* tm, hipStream_t s);
 
 // Weights + ensemble transform + RTPP/RTPS + round_off for points [p0,p1).
 template <typename T>

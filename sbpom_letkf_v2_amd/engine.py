@@ -85,7 +85,11 @@ class Engine:
     def stats(self) -> dict:
         s = _lib.Stats()
         check(_lib.lib().letkf_get_stats(self.h, C.byref(s)))
-        return {n: getattr(s, n) for n, _ in s._fields_}
+        out = {}
+        for n, _ in s._fields_:
+            v = getattr(s, n)
+            out[n] = list(v) if n in ("ms_eigk", "n_eigk") else v
+        return out
 
     # -------------------------------------------------------- host arrays
     def da_main_host(self, pr):
