@@ -175,6 +175,30 @@ int letkf_bin_obs(letkf_handle_t* h, int32_t nlon, int32_t nlat,
 int letkf_aoei(letkf_handle_t* h, int32_t nobs, void* err, const void* dep,
                const void* hxfsprd);
 
+/* Adjoint of letkf_da_main_device (no counterpart in the reference; for
+ * gradient-based uses of the analysis: observation impact, tuning of the
+ * inflation and of the observation errors, torch.autograd).  DEVICE arrays;
+ * grid, obs, fcst and the handle's parameters must be those of the forward
+ * call.  Given ga3/ga2 = dL/d(anal3d, anal2d), writes
+ *   g_dep[nobs]    dL/d obs->dep          (innovations, i.e. obs values)
+ *   g_logvar[nobs] dL/d log(obs->err^2)   (per-obs log error variance)
+ *   g_scalars[2]   dL/d alpha_rtpp, dL/d cov_infl_mul.
+ * The per-point eigen-decompositions are recomputed (FP64, K x K form);
+ * round_off clamps pass no gradient.  Needs real_kind 8, nmem <= 64 and
+ * alpha_rtps = 0.  Work is queued on `stream` (NULL: the handle's stream)
+ * and the call returns after it completes. */
+int letkf_da_adjoint_device(letkf_handle_t* h, const letkf_grid_t* grid,
+                            const letkf_obs_t* obs, const void* fcst3d,
+                            const void* fcst2d, const void* ga3,
+                            const void* ga2, double* g_dep, double* g_logvar,
+                            double* g_scalars, void* stream);
+
+/* Replace the parameters of a handle (e.g. cov_infl_mul, alpha_rtpp,
+ * precision, form) keeping its workspaces; sizes and real_kind must not
+ * change.  No counterpart in the reference (its parameters are PARAMETERs
+ * of common_setting.f90:71-131); for callers that vary them between calls. */
+int letkf_set_params(letkf_handle_t* h, const letkf_params_t* p);
+
 int letkf_get_stats(const letkf_handle_t* h, letkf_stats_t* out);
 const char* letkf_last_error(void);
 

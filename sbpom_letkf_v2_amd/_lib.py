@@ -22,8 +22,8 @@ FORM_PLAIN, FORM_BATCHED = 0, 1
 # every symbol include/letkf_engine.h declares
 EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_da_main", "letkf_da_main_device", "letkf_core_batched",
-           "letkf_bin_obs", "letkf_aoei", "letkf_get_stats",
-           "letkf_last_error")
+           "letkf_bin_obs", "letkf_aoei", "letkf_da_adjoint_device",
+           "letkf_set_params", "letkf_get_stats", "letkf_last_error")
 
 
 class Params(C.Structure):
@@ -87,6 +87,9 @@ def lib():
                                         vp, vp, vp, vp, vp, vp]
         if hasattr(L, "letkf_aoei"):
             L.letkf_aoei.argtypes = [vp, C.c_int32, vp, vp, vp]
+        L.letkf_da_adjoint_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp,
+                                              vp, vp, vp, vp]
+        L.letkf_set_params.argtypes = [vp, C.POINTER(Params)]
         L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.letkf_last_error.restype = C.c_char_p
         for n in EXPORTS:

@@ -605,5 +605,6 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D, dou
 }
 
 #include "eigreg.inc"
+#include "adjoint.inc"
 
 }  // namespace letkf

@@ -513,6 +513,102 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   return status_to_rc(status);
 }
 
+// Adjoint of run_device<double> (see adjoint.inc): search and FP64 Gram in
+// the K x K form per chunk, then the per-point backward kernels.
+static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const double* f3,
+                       const double* f2, const double* b3, const double* b2, double* gdep, double* glogvar,
+                       double* gsc, hipStream_t s) {
+  const letkf_params_t& p = h->prm;
+  const int K = p.nmem;
+  KParams kp = make_kparams(p, o->nobs);
+  const long long P = kp.P;
+  HIP_TRY(hipMemsetAsync(gsc, 0, 2 * sizeof(double), s));
+  if (o->nobs > 0) {
+    HIP_TRY(hipMemsetAsync(gdep, 0, sizeof(double) * o->nobs, s));
+    HIP_TRY(hipMemsetAsync(glogvar, 0, sizeof(double) * o->nobs, s));
+  }
+  if (o->nobs == 0) {  // analysis = forecast (common_da.f90:57-61): no dependence
+    HIP_TRY(hipStreamSynchronize(s));
+    return LETKF_OK;
+  }
+  Tables<double> tb;
+  tb.lon1 = (const double*)g->lon1;
+  tb.lat1 = (const double*)g->lat1;
+  tb.ri1 = (const double*)g->ri1;
+  tb.rj1 = (const double*)g->rj1;
+  tb.dlon1 = (const double*)g->dlon1;
+  tb.dlat1 = (const double*)g->dlat1;
+  tb.phi1 = (const double*)g->phi1;
+  tb.depth1 = (const double*)g->depth1;
+  tb.nobsgrd = o->nobsgrd;
+  tb.elm = o->elm;
+  tb.olon = (const double*)o->lon;
+  tb.olat = (const double*)o->lat;
+  tb.olev = (const double*)o->lev;
+  tb.oerr = (const double*)o->err;
+  tb.odep = (const double*)o->dep;
+  HIP_TRY(h->nloc.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->loff.ensure(sizeof(int64_t) * (P + 1)));
+  HIP_TRY(h->actidx.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->act_p.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->scan_tmp.ensure(scan_tmp_bytes(P)));
+  HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
+  HIP_TRY(h->adj_part.ensure(sizeof(double) * 2 * (size_t)P));
+  HIP_TRY(hipMemsetAsync(h->status.ptr, 0, sizeof(int) * 4, s));
+  HIP_TRY(hipMemsetAsync(h->adj_part.ptr, 0, sizeof(double) * 2 * (size_t)P, s));
+  HIP_TRY(launch_transpose_obs<double>(o->nobs, K, (const double*)o->hdxf, h->yT.as<double>(), s));
+  HIP_TRY(launch_search_count<double>(kp, tb, h->nloc.as<int32_t>(), s));
+  HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
+                             h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
+  // chunks with a K x K Gram block per active slot (~2 GiB)
+  long long PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
+  PC = std::max(1024LL, std::min(PC, P));
+  const long long nch = (P + PC - 1) / PC;
+  std::vector<long long> bpts(nch + 1);
+  for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
+  HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 5)));
+  long long* d_pts = h->bounds.as<long long>();
+  long long* d_out = d_pts + (nch + 1);
+  HIP_TRY(hipMemcpyAsync(d_pts, bpts.data(), sizeof(long long) * (nch + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_gather_bounds, dim3((unsigned)((nch + 1 + 255) / 256)), dim3(256), 0, s, (int)(nch + 1),
+                     d_pts, h->loff.as<int64_t>(), h->actidx.as<int32_t>(), d_out);
+  HIP_TRY(hipGetLastError());
+  std::vector<long long> bv(2 * (nch + 1));
+  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * 2 * (nch + 1), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  long long maxL = 0, maxA = 0;
+  for (long long c = 0; c < nch; ++c) {
+    maxL = std::max(maxL, bv[2 * (c + 1)] - bv[2 * c]);
+    maxA = std::max(maxA, bv[2 * (c + 1) + 1] - bv[2 * c + 1]);
+  }
+  HIP_TRY(h->lobs.ensure(sizeof(int32_t) * (maxL + 1)));
+  HIP_TRY(h->lrloc.ensure(sizeof(double) * (maxL + 1)));
+  HIP_TRY(h->G.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
+  HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+  for (long long c = 0; c < nch; ++c) {
+    const long long p0 = bpts[c], p1 = bpts[c + 1];
+    const long long lbase = bv[2 * c];
+    const int a0 = (int)bv[2 * c + 1];
+    const int na = (int)(bv[2 * (c + 1) + 1] - a0);
+    if (na > 0) {
+      HIP_TRY(launch_search_fill<double>(kp, tb, p0, p1, h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
+                                         h->lrloc.as<double>(), s));
+      HIP_TRY(launch_gram_fp64<double>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
+                                       lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
+                                       h->G.as<double>(), h->bvec.as<double>(), 0, s));
+    }
+    HIP_TRY(launch_adjoint_points(kp, tb, f3, f2, b3, b2, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), lbase,
+                                  h->lobs.as<int32_t>(), h->lrloc.as<double>(), h->yT.as<double>(),
+                                  h->G.as<double>(), h->bvec.as<double>(), h->act_p.as<int32_t>(), a0, na, p0, p1,
+                                  gdep, glogvar, h->adj_part.as<double>(), h->status.as<int>(), s));
+  }
+  HIP_TRY(launch_adjoint_reduce(h->adj_part.as<double>(), P, gsc, s));
+  int status = 0;
+  HIP_TRY(hipMemcpyAsync(&status, h->status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return status_to_rc(status);
+}
+
 static int check_call(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
                       const void* f2, void* a3, void* a2) {
   if (!h) return set_err(LETKF_ERR_ARG, "handle is NULL");
@@ -539,6 +635,34 @@ extern "C" int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* g, co
   if (h->prm.real_kind == 8)
     return run_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s);
   return run_device<float>(h, g, o, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s);
+}
+
+extern "C" int letkf_da_adjoint_device(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o,
+                                       const void* f3, const void* f2, const void* ga3, const void* ga2,
+                                       double* g_dep, double* g_logvar, double* g_scalars, void* stream) {
+  int rc = check_call(h, g, o, f3, f2, (void*)ga3, (void*)ga2);
+  if (rc) return rc;
+  const letkf_params_t& p = h->prm;
+  if (p.real_kind != 8) return set_err(LETKF_ERR_ARG, "the adjoint needs real_kind 8");
+  if (p.nmem > 64) return set_err(LETKF_ERR_ARG, "the adjoint supports nmem <= 64 (got %d)", p.nmem);
+  if (p.alpha_rtps != 0.0) return set_err(LETKF_ERR_ARG, "the adjoint needs alpha_rtps = 0 (RTPS is not differentiated)");
+  if (!g_scalars || (o->nobs > 0 && (!g_dep || !g_logvar))) return set_err(LETKF_ERR_ARG, "gradient array is NULL");
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  return run_adjoint(h, g, o, (const double*)f3, (const double*)f2, (const double*)ga3, (const double*)ga2, g_dep,
+                     g_logvar, g_scalars, s);
+}
+
+extern "C" int letkf_set_params(letkf_handle_t* h, const letkf_params_t* p) {
+  if (!h) return set_err(LETKF_ERR_ARG, "handle is NULL");
+  int rc = check_params(p);
+  if (rc) return rc;
+  const letkf_params_t& q = h->prm;
+  if (p->nmem != q.nmem || p->nlon != q.nlon || p->nlat != q.nlat || p->nlev != q.nlev || p->nij1 != q.nij1 ||
+      p->nv3d != q.nv3d || p->nv2d != q.nv2d || p->real_kind != q.real_kind)
+    return set_err(LETKF_ERR_ARG, "letkf_set_params: sizes and real_kind must not change");
+  h->prm = *p;
+  return LETKF_OK;
 }
 
 extern "C" int letkf_da_main(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,

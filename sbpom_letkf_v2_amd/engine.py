@@ -114,22 +114,34 @@ class Engine:
         check(_lib.lib().letkf_da_main(self.h, C.byref(g), C.byref(o), vp(f3), vp(f2), vp(a3), vp(a2)))
         return a3, a2
 
+    def set_params(self, params: _lib.Params):
+        """letkf_set_params: new scalars / precision / form, same sizes."""
+        check(_lib.lib().letkf_set_params(self.h, C.byref(params)))
+        self.params = params
+
     # ------------------------------------------------------ device arrays
-    def da_main_device(self, pr, anal3d=None, anal2d=None, stream=None):
+    def _device_structs(self, pr):
         import torch
         dt = torch.float32 if self.params.real_kind == 4 else torch.float64
-        for n in ("lon1", "fcst3d", "obshdxf"):
+        for n in ("lon1", "fcst3d", "obshdxf", "obsdep", "obserr"):
             a = getattr(pr, n)
             if not (_is_torch(a) and a.is_cuda):
                 raise TypeError(f"{n} must be a torch tensor on the GPU")
             if a.dtype != dt:
                 raise TypeError(f"{n} has dtype {a.dtype}, real_kind needs {dt}")
-        ptr = lambda a: C.c_void_p(a.data_ptr()) if a is not None and a.numel() else None
-        g = _lib.Grid(*(ptr(getattr(pr, n).contiguous()) for n in (
+            if not a.is_contiguous():
+                raise ValueError(f"{n} must be contiguous")
+        g = _lib.Grid(*(_ptr(getattr(pr, n)) for n in (
             "lon1", "lat1", "ri1", "rj1", "dlon1", "dlat1", "phi1", "depth1")))
-        o = _lib.Obs(int(pr.obselm.numel()), ptr(pr.nobsgrd), ptr(pr.obselm),
-                     *(ptr(getattr(pr, n)) for n in ("obslon", "obslat", "obslev", "obserr",
-                                                     "obsdep", "obshdxf")))
+        o = _lib.Obs(int(pr.obselm.numel()), _ptr(pr.nobsgrd), _ptr(pr.obselm),
+                     *(_ptr(getattr(pr, n)) for n in ("obslon", "obslat", "obslev", "obserr",
+                                                      "obsdep", "obshdxf")))
+        return g, o
+
+    def da_main_device(self, pr, anal3d=None, anal2d=None, stream=None):
+        import torch
+        g, o = self._device_structs(pr)
+        ptr = _ptr
         if anal3d is None:
             anal3d = torch.empty_like(pr.fcst3d)
         if anal2d is None:
@@ -138,6 +150,29 @@ class Engine:
         check(_lib.lib().letkf_da_main_device(self.h, C.byref(g), C.byref(o), ptr(pr.fcst3d),
                                               ptr(pr.fcst2d), ptr(anal3d), ptr(anal2d), s))
         return anal3d, anal2d
+
+    def da_adjoint_device(self, pr, g3, g2, stream=None):
+        """letkf_da_adjoint_device: (dL/d obsdep, dL/d log obserr^2,
+        [dL/d alpha_rtpp, dL/d cov_infl_mul]) for upstream gradients g3, g2 of
+        (anal3d, anal2d); pr and the handle's parameters as in the forward call."""
+        import torch
+        g, o = self._device_structs(pr)
+        dev = pr.fcst3d.device
+        nobs = int(pr.obselm.numel())
+        gd = torch.zeros(nobs, dtype=torch.float64, device=dev)
+        gl = torch.zeros(nobs, dtype=torch.float64, device=dev)
+        gs = torch.zeros(2, dtype=torch.float64, device=dev)
+        g3 = g3.contiguous()
+        g2 = g2.contiguous() if g2 is not None else None
+        s = C.c_void_p(stream) if stream else None
+        check(_lib.lib().letkf_da_adjoint_device(self.h, C.byref(g), C.byref(o), _ptr(pr.fcst3d),
+                                                 _ptr(pr.fcst2d), _ptr(g3), _ptr(g2), _ptr(gd), _ptr(gl),
+                                                 _ptr(gs), s))
+        return gd, gl, gs
+
+
+def _ptr(a):
+    return C.c_void_p(a.data_ptr()) if a is not None and a.numel() else None
 
 
 def da_main(pr, precision="fp64", form="plain"):
