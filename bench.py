@@ -46,7 +46,94 @@ def parse():
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="differentiable workload (default c3): time forward + hand-written adjoint and "
+                         "check it by central finite differences")
+    ap.add_argument("--fd-coords", type=int, default=200)
     return ap.parse_args()
+
+
+def run_adjoint(args):
+    """Loss = squared misfit of the analysis mean to the synthetic truth;
+    gradient w.r.t. obs values, per-obs log sigma^2, RTPP alpha and rho."""
+    import numpy as np
+    import torch
+    from sbpom_letkf_v2_amd import autograd as ag
+    from sbpom_letkf_v2_amd import synth
+    wl = args.workload if args.workload != "c2" else "c3"
+    log(f"generating workload {wl} on the GPU")
+    pr = synth.config(wl, device="cuda", real_kind=8, keep_truth=True, progress=log)
+    meta = pr.meta
+    K, nv3 = meta["nmem"], meta["nv3d"]
+    P = meta["nlon"] * meta["nlat"] * meta["nlev"]
+    t3 = pr.truth3d.reshape(nv3, meta["nlev"], -1)
+
+    def loss(a3, a2):
+        return ((a3.mean(1) - t3) ** 2).sum()
+
+    st = ag.AnalysisState(pr, "fp64", "plain")
+    theta = [pr.obsdep.clone(), torch.log(pr.obserr.clone() ** 2),
+             torch.tensor(float(pr.params["alpha_rtpp"]), dtype=torch.float64, device="cuda"),
+             torch.tensor(float(pr.params.get("cov_infl_mul", 1.0)), dtype=torch.float64, device="cuda")]
+
+    def value(th):
+        with torch.no_grad():
+            a3, a2 = ag.analysis(st, *th)
+            return loss(a3, a2).item()
+
+    fwd, bwd = [], []
+    for s in range(args.warmup + args.steps):
+        th = [t.clone().requires_grad_(True) for t in theta]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        a3, a2 = ag.analysis(st, *th)
+        L = loss(a3, a2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if s >= args.warmup:
+            fwd.append(t1 - t0)
+            bwd.append(t2 - t1)
+        log(f"  step {s}: forward {1e3 * (t1 - t0):.1f} ms, adjoint {1e3 * (t2 - t1):.1f} ms")
+    g = [t.grad.detach() for t in th]
+    gmax = max(float(x.abs().max()) for x in g)
+    rng = np.random.default_rng(20261203)
+    h = 1e-4
+    errs = []
+    nc = args.fd_coords
+    picks = [(0, int(j)) for j in rng.choice(theta[0].numel(), nc // 2 - 1, False)] + \
+            [(1, int(j)) for j in rng.choice(theta[1].numel(), nc - nc // 2 - 1, False)] + [(2, None), (3, None)]
+    t0 = time.perf_counter()
+    for which, j in picks:
+        tp = [t.clone() for t in theta]
+        tm = [t.clone() for t in theta]
+        if j is None:
+            tp[which] += h
+            tm[which] -= h
+            ad = float(g[which])
+        else:
+            tp[which][j] += h
+            tm[which][j] -= h
+            ad = float(g[which][j])
+        fd = (value(tp) - value(tm)) / (2 * h)
+        errs.append(abs(fd - ad) / max(abs(ad), 1e-3 * gmax))
+    log(f"  FD check on {len(picks)} coordinates in {time.perf_counter() - t0:.0f}s: "
+        f"max rel err {max(errs):.2e}")
+    st.close()
+    fm, bm = sum(fwd) / len(fwd), sum(bwd) / len(bwd)
+    out = {"metric": "adjoint of the analysis: analysed grid pts/s (forward + adjoint)",
+           "value": P / (fm + bm), "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * (fm + bm), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{wl}: {meta['nlon']}x{meta['nlat']}x{meta['nlev']} K={K} nv3d={nv3} "
+                                  f"nobs={meta['nobs']}, loss = squared misfit of the analysis mean to the truth"},
+           "adjoint": {"forward_ms": 1e3 * fm, "adjoint_ms": 1e3 * bm, "fd_coords": len(picks), "fd_h": h,
+                       "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
+                       "grad_max": {"dep": float(g[0].abs().max()), "logvar": float(g[1].abs().max()),
+                                    "alpha": float(g[2]), "rho": float(g[3])}}}
+    print(json.dumps(out), flush=True)
 
 
 def run_form(prd, precision, form, steps, warmup):
@@ -166,6 +253,9 @@ def main():
               "by grid points across GPUs with no exchange inside the path -- run N=1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.adjoint:
+        run_adjoint(args)
+        return
     import numpy as np
     import torch
     import pyoracle as po
