@@ -150,8 +150,8 @@ def run_form(prd, precision, form, steps, warmup):
         log(f"  warmup {w}: {time.perf_counter() - t:.2f}s {eng.stats()}")
     torch.cuda.synchronize()
     tot = {k: 0.0 for k in ("ms_search", "ms_gram", "ms_eig", "ms_transform", "ms_total")}
-    kms = [0.0] * 20
-    kn = [0] * 20
+    kms = [0.0] * 25
+    kn = [0] * 25
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):
@@ -159,7 +159,7 @@ def run_form(prd, precision, form, steps, warmup):
         st = eng.stats()
         for k in tot:
             tot[k] += st[k]
-        for q in range(20):
+        for q in range(25):
             kms[q] += st["ms_eigk"][q]
             kn[q] += st["n_eigk"][q]
         log(f"  step {s}: {st['ms_total']:.1f} ms")
@@ -168,9 +168,9 @@ def run_form(prd, precision, form, steps, warmup):
     st = eng.stats()
     eng.close()
     stage = {k: v / steps for k, v in tot.items()}
-    stage["eig_kernels"] = {EIG_KINDS[q // 4] + f"<{16 * (q % 4 + 1)}>": {"ms": kms[q] / steps,
-                                                                          "launches": kn[q] / steps}
-                            for q in range(20) if kn[q]}
+    stage["eig_kernels"] = {EIG_KINDS[q // 5] + f"<{(16, 32, 48, 64, 128)[q % 5]}>": {"ms": kms[q] / steps,
+                                                                                    "launches": kn[q] / steps}
+                            for q in range(25) if kn[q]}
     return a3, a2, wall / steps, stage, st
 
 

@@ -115,12 +115,13 @@ typedef struct letkf_stats {
                            /* kernel: fill, Gram, transform)            */
   double eig_flops;        /* (4/3) m^3 summed over active points, m the */
                            /* eigenproblem size (Householder reduction) */
-  double ms_eigk[20];      /* stage_timing 2, K <= 64: summed time of the */
-  int64_t n_eigk[20];      /* eigensolve kernels and their launch counts, */
-                           /* index 4*kind + bucket; kind 0 tridiagonal- */
-                           /* isation, 1 QL, 2 Q^T on the vectors, 3     */
-                           /* rotation replay, 4 Q and the update;       */
-                           /* bucket 0..3 = size <= 16, 32, 48, 64       */
+  double ms_eigk[25];      /* stage_timing 2, K <= 128: summed time of   */
+  int64_t n_eigk[25];      /* the eigensolve kernels and their launch     */
+                           /* counts, index 5*kind + bucket; kind 0      */
+                           /* tridiagonalisation, 1 QL, 2 Q^T on the     */
+                           /* vectors, 3 rotation replay, 4 Q and the    */
+                           /* update; bucket 0..4 = size <= 16, 32, 48,  */
+                           /* 64, 128                                    */
 } letkf_stats_t;
 
 typedef struct letkf_handle letkf_handle_t;

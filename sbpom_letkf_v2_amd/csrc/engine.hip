@@ -199,16 +199,19 @@ __global__ void k_point_stats(long long P, const int32_t* nloc, int K, int dual,
 }
 
 // Size buckets of the register eigensolve: problem size s (K, or nobsl < K
-// in the dual form) goes to bucket (s-1)/16, i.e. nb = 16, 32, 48, 64.
-constexpr int kNQ = 4;
+// in the dual form) goes to bucket min((s-1)/16, 4), i.e. nb = 16, 32, 48,
+// 64, 128.
+constexpr int kNQ = 5;
+static int bucket_nb(int q) { return q < 4 ? 16 * (q + 1) : 128; }
 
 __device__ __forceinline__ int slot_size(int n, int K, int dual) { return dual ? (n < K ? n : K) : K; }
+__device__ __forceinline__ int size_bucket(int m) { return (m - 1) / 16 < 4 ? (m - 1) / 16 : 4; }
 
 __global__ void k_bucket_count(long long P, const int32_t* nloc, int K, int dual, long long PC, int32_t* hist) {
   for (long long p = blockIdx.x * 256LL + threadIdx.x; p < P; p += 256LL * gridDim.x) {
     const int n = nloc[p];
     if (n == 0) continue;
-    const int q = (slot_size(n, K, dual) - 1) / 16;
+    const int q = size_bucket(slot_size(n, K, dual));
     atomicAdd(&hist[(p / PC) * kNQ + q], 1);
   }
 }
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(256) k_bucket_scatter(long long P, const int32
     int q = -1;
     if (p < P) {
       const int n = nloc[p];
-      if (n > 0) q = (slot_size(n, K, dual) - 1) / 16;
+      if (n > 0) q = size_bucket(slot_size(n, K, dual));
     }
     // all points of a 256-block lie in one chunk when PC is a multiple of 256
     const long long c = p0 / PC;
@@ -329,7 +332,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   // eigensolve (K <= 64) wants many matrices per launch, so its chunks are
   // sized by a ~24 GB workspace budget; the LDS path keeps 2 GiB per block.
   long long PC;
-  if (K <= 64) {
+  if (K <= 128) {
     const double ncol = 1.0 + p.nv3d + p.nv2d;
     const double per_slot = 16.0 * K * K + 8.0 * K * ncol + 16.0 * eigr_rot_capacity(K) +
                             (double)eigr_ix_pitch(K) + 8.0 * eigr_sweep_capacity(K) + 64.0 * K + 8.0;
@@ -367,7 +370,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   st.active_points = bv[2 * nch + 1];
   // register eigensolve for K <= 64: bucket the active slots of every chunk
   // by problem size; bucket (c, q) occupies list[boff[c*kNQ+q] ...)
-  const bool reg_eig = K <= 64;
+  const bool reg_eig = K <= 128;
   std::vector<int32_t> hist, boff;
   if (reg_eig) {
     const size_t nh = (size_t)nch * kNQ;
@@ -453,7 +456,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
         for (int q = 0; q < kNQ; ++q) {
           const int cnt = hist[(size_t)c * kNQ + q];
           if (cnt == 0) continue;
-          HIP_TRY(launch_eigr_update<T>(16 * (q + 1), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
+          HIP_TRY(launch_eigr_update<T>(bucket_nb(q), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
                                         h->blist.as<int32_t>() + boff[(size_t)c * kNQ + q], cnt, h->G.as<double>(),
                                         h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
                                         h->logc.as<double>(), h->logx.as<uint8_t>(), h->logi.as<int32_t>(),
