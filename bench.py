@@ -299,7 +299,7 @@ def main():
         return ms, n
 
     tri_ms, tri_n = family("k_eigr_tridiag")
-    gram_kernel = "k_gram_fp64" if args.precision == "fp64" or K > 64 else "k_gram_mfma"
+    gram_kernel = "k_gram_fp64" if args.precision == "fp64" or K > 128 else "k_gram_mfma"
     kernels = {
         # Gram: 2 n K^2 (primal) or 2 n^2 K (dual) flops per point, one launch per chunk
         gram_kernel: dict(ms=stage["ms_gram"], launches=nch, work=st["gram_flops"], unit="TFLOP/s",

@@ -325,11 +325,9 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
                              h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
   HIP_TRY(mark(1));
 
-  // chunks of consecutive points sized so that the per-point K x K
-  // workspaces (G, V) stay within ~2 GiB each
   // Chunk of consecutive points: its active slots own a K x K Gram and
   // eigenvector block and the eigensolve's rotation log.  The register
-  // eigensolve (K <= 64) wants many matrices per launch, so its chunks are
+  // eigensolve (K <= 128) wants many matrices per launch, so its chunks are
   // sized by a ~24 GB workspace budget; the LDS path keeps 2 GiB per block.
   long long PC;
   if (K <= 128) {
@@ -368,7 +366,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   }
   st.local_obs = bv[2 * nch];
   st.active_points = bv[2 * nch + 1];
-  // register eigensolve for K <= 64: bucket the active slots of every chunk
+  // register eigensolve for K <= 128: bucket the active slots of every chunk
   // by problem size; bucket (c, q) occupies list[boff[c*kNQ+q] ...)
   const bool reg_eig = K <= 128;
   std::vector<int32_t> hist, boff;
@@ -441,7 +439,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     }
     HIP_TRY(mark(e0 + 1));
     if (na > 0) {
-      if (p.precision != LETKF_PREC_FP64 && K <= 64)
+      if (p.precision != LETKF_PREC_FP64 && K <= 128)
         HIP_TRY(launch_gram_mixed<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
                                      h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0,
                                      na, h->G.as<double>(), h->bvec.as<double>(), dual, s));

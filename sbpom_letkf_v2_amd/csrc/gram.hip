@@ -236,9 +236,10 @@ template hipError_t launch_gram_fp64<double>(const KParams&, const Tables<double
                                              const double*, int, int, double*, double*, int, hipStream_t);
 
 // ------------------------------------------------------------------------
-// Mixed-precision Gram on MFMA (K <= 64): the K x K (or dual n x n) product
-// runs as four 32x32 output tiles, one per wavefront, with FP32
-// accumulation:
+// Mixed-precision Gram on MFMA (K <= 128): the K x K (or dual n x n)
+// product runs as 32x32 output tiles with FP32 accumulation; the four
+// wavefronts of a block split the OUT x OUT output (OUT = 64 or 128) into
+// quadrants of (OUT/64)^2 tiles each, skipping tiles outside the problem:
 //   prec FP32   : v_mfma_f32_32x32x2f32 on FP32-rounded operands;
 //   prec BF16X3 : operands split x = hi + lo (bf16 each), three products
 //                 hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_bf16.
@@ -252,21 +253,26 @@ typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMfPanel = 32;  // obs (primal) or members (dual) per LDS panel
 
-template <int PREC, typename Src>
+template <int PREC, int OUT, typename Src>
 __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, double* __restrict__ G,
                                                    double* __restrict__ bvec, int dual) {
-  // panel: rows = summation index (obs or member), cols = output index (64)
-  __shared__ float As[kMfPanel][64 + 4];
-  __shared__ float Bs[kMfPanel][64 + 4];
-  __shared__ double sws[64];
-  __shared__ int ios[64];
+  constexpr int TW = OUT / 64;  // tiles per wavefront along each dimension
+  // panel: rows = summation index (obs or member), cols = output index
+  __shared__ float As[kMfPanel][OUT + 4];
+  __shared__ float Bs[kMfPanel][OUT + 4];
+  __shared__ double sws[OUT];
+  __shared__ int ios[OUT];
   const int n = src.begin(blockIdx.x);
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int r0 = (wv >> 1) * 32, c0 = (wv & 1) * 32;  // output tile of this wave
+  const int r0 = (wv >> 1) * (OUT / 2), c0 = (wv & 1) * (OUT / 2);  // quadrant of this wave
   const bool dm = dual && n < K;
-  const int N = dm ? n : K;                         // output dimension
-  const int S = dm ? K : n;                         // summation length
-  mf_v16 acc = {};
+  const int N = dm ? n : K;  // output dimension
+  const int S = dm ? K : n;  // summation length
+  mf_v16 acc[TW][TW];
+#pragma unroll
+  for (int a = 0; a < TW; ++a)
+#pragma unroll
+    for (int c = 0; c < TW; ++c) acc[a][c] = mf_v16{};
   if (dm) {
     for (int l = t; l < n; l += 256) {
       int io;
@@ -287,10 +293,16 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
     bvec[(long long)blockIdx.x * K + t] = bacc;
   }
   __syncthreads();
+  // tiles of this wave inside the N x N output
+  bool live[TW][TW];
+#pragma unroll
+  for (int a = 0; a < TW; ++a)
+#pragma unroll
+    for (int c = 0; c < TW; ++c) live[a][c] = r0 + 32 * a < N && c0 + 32 * c < N;
   for (int s0 = 0; s0 < S; s0 += kMfPanel) {
     const int ps = min(kMfPanel, S - s0);
-    for (int q = t; q < kMfPanel * 64; q += 256) {
-      const int row = q >> 6, col = q & 63;
+    for (int q = t; q < kMfPanel * OUT; q += 256) {
+      const int row = q / OUT, col = q % OUT;
       double av = 0.0, bv = 0.0;
       if (row < ps && col < N) {
         if (dm) {
@@ -311,38 +323,71 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
     if (PREC == LETKF_PREC_FP32) {
       for (int k = 0; k < ps; k += 2) {
         const int kk = k + (lane >> 5);
-        const float a = kk < ps ? As[kk][r0 + (lane & 31)] : 0.f;
-        const float b = kk < ps ? Bs[kk][c0 + (lane & 31)] : 0.f;
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        float av[TW], bv[TW];
+#pragma unroll
+        for (int a = 0; a < TW; ++a) {
+          av[a] = kk < ps ? As[kk][r0 + 32 * a + (lane & 31)] : 0.f;
+          bv[a] = kk < ps ? Bs[kk][c0 + 32 * a + (lane & 31)] : 0.f;
+        }
+#pragma unroll
+        for (int a = 0; a < TW; ++a)
+#pragma unroll
+          for (int c = 0; c < TW; ++c)
+            if (live[a][c]) acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[c], acc[a][c], 0, 0, 0);
       }
     } else {
       for (int k = 0; k < ps; k += 16) {
-        mf_bf8 ah, al, bh, bl;
+        mf_bf8 ah[TW], al[TW], bh[TW], bl[TW];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int kk = k + 8 * (lane >> 5) + q;
-          const float a = kk < ps ? As[kk][r0 + (lane & 31)] : 0.f;
-          const float b = kk < ps ? Bs[kk][c0 + (lane & 31)] : 0.f;
-          const __bf16 a1 = (__bf16)a, b1 = (__bf16)b;
-          ah[q] = a1;
-          bh[q] = b1;
-          al[q] = (__bf16)(a - (float)a1);
-          bl[q] = (__bf16)(b - (float)b1);
+        for (int a = 0; a < TW; ++a) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int kk = k + 8 * (lane >> 5) + q;
+            const float x = kk < ps ? As[kk][r0 + 32 * a + (lane & 31)] : 0.f;
+            const float y = kk < ps ? Bs[kk][c0 + 32 * a + (lane & 31)] : 0.f;
+            const __bf16 x1 = (__bf16)x, y1 = (__bf16)y;
+            ah[a][q] = x1;
+            bh[a][q] = y1;
+            al[a][q] = (__bf16)(x - (float)x1);
+            bl[a][q] = (__bf16)(y - (float)y1);
+          }
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < TW; ++a)
+#pragma unroll
+          for (int c = 0; c < TW; ++c) {
+            if (!live[a][c]) continue;
+            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
+          }
       }
     }
     __syncthreads();
   }
   double* Gp = G + (long long)blockIdx.x * K * K;
-  const int col = c0 + (lane & 31);
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = r0 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-    if (row < N && col < N) Gp[row * N + col] = (double)acc[r] + (!dm && row == col ? diag : 0.0);
-  }
+  for (int a = 0; a < TW; ++a)
+#pragma unroll
+    for (int c = 0; c < TW; ++c) {
+      const int col = c0 + 32 * c + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = r0 + 32 * a + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        if (row < N && col < N) Gp[row * N + col] = (double)acc[a][c][r] + (!dm && row == col ? diag : 0.0);
+      }
+    }
+}
+
+template <int PREC, typename T>
+static void launch_gram_mixed_p(int K, double diag, const ObsSource<T>& src, int na, double* G, double* bvec,
+                                int dual, hipStream_t s) {
+  if (K <= 64)
+    hipLaunchKernelGGL((k_gram_mfma<PREC, 64, ObsSource<T>>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec,
+                       dual);
+  else
+    hipLaunchKernelGGL((k_gram_mfma<PREC, 128, ObsSource<T>>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec,
+                       dual);
 }
 
 template <typename T>
@@ -351,15 +396,13 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
                              const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
                              hipStream_t s) {
   if (na <= 0) return hipSuccess;
-  if (kp.K > 64) return hipErrorInvalidValue;
+  if (kp.K > 128) return hipErrorInvalidValue;
   ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
   if (prec == LETKF_PREC_FP32)
-    hipLaunchKernelGGL((k_gram_mfma<LETKF_PREC_FP32, ObsSource<T>>), dim3(na), dim3(256), 0, s, kp.K, diag, src,
-                       G, bvec, dual);
+    launch_gram_mixed_p<LETKF_PREC_FP32, T>(kp.K, diag, src, na, G, bvec, dual, s);
   else
-    hipLaunchKernelGGL((k_gram_mfma<LETKF_PREC_BF16X3, ObsSource<T>>), dim3(na), dim3(256), 0, s, kp.K, diag,
-                       src, G, bvec, dual);
+    launch_gram_mixed_p<LETKF_PREC_BF16X3, T>(kp.K, diag, src, na, G, bvec, dual, s);
   return hipGetLastError();
 }
 template hipError_t launch_gram_mixed<float>(int, const KParams&, const Tables<float>&, const double*,

@@ -218,13 +218,16 @@ def test_aoei_gpu_vs_oracle():
     assert np.array_equal(te.cpu().numpy(), po.aoei(e, d, s))
 
 
-@pytest.mark.parametrize("precision,form", [("fp32", "batched"), ("bf16x3", "batched"), ("fp32", "plain")])
-def test_mixed_precision_gram_close_to_fp64(precision, form):
+@pytest.mark.parametrize("precision,form,K,nobs", [("fp32", "batched", 48, 900), ("bf16x3", "batched", 48, 900),
+                                                   ("fp32", "plain", 48, 900), ("fp32", "batched", 100, 1800),
+                                                   ("bf16x3", "batched", 128, 2000), ("fp32", "plain", 100, 1800)])
+def test_mixed_precision_gram_close_to_fp64(precision, form, K, nobs):
     """Gram on MFMA (FP32, or BF16 hi/lo split) with FP64 eigensolve: the
     analysis must stay within 1e-4 of the FP64 oracle relative to the field
-    scale (FP32 products keep ~7 digits; the innovation term stays FP64)."""
-    _say(f"case {precision}/{form}")
-    pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=48, nobs=900, seed=61, alpha_rtpp=0.7,
+    scale (FP32 products keep ~7 digits; the innovation term stays FP64).
+    K > 64 runs the 128-wide output tiling, partly filled for K = 100."""
+    _say(f"case {precision}/{form} K={K} nobs={nobs}")
+    pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=K, nobs=nobs, seed=61, alpha_rtpp=0.7,
                             real_kind=8, sigma_h_cells=1.0, sigma_v=1.0)
     a3, a2, a3o, a2o, nl, st = _run_both(pr, precision=precision, form=form)
     e3 = per_var_err(a3, a3o, axis_var=3)
