@@ -41,6 +41,27 @@ __device__ __forceinline__ double wave_sum(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Two wavefront sums interleaved (one reduction latency for both).
+__device__ __forceinline__ void wave_sum2(double& a, double& b) {
+  a += dpp_d<0xB1, 0xF>(a);
+  b += dpp_d<0xB1, 0xF>(b);
+  a += dpp_d<0x4E, 0xF>(a);
+  b += dpp_d<0x4E, 0xF>(b);
+  a += dpp_d<0x141, 0xF>(a);
+  b += dpp_d<0x141, 0xF>(b);
+  a += dpp_d<0x140, 0xF>(a);
+  b += dpp_d<0x140, 0xF>(b);
+  a += dpp_d<0x142, 0xA>(a);
+  b += dpp_d<0x142, 0xA>(b);
+  a += dpp_d<0x143, 0xC>(a);
+  b += dpp_d<0x143, 0xC>(b);
+  const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+  const int xl = __builtin_amdgcn_readlane((int)x, 63), xh = __builtin_amdgcn_readlane((int)(x >> 32), 63);
+  const int yl = __builtin_amdgcn_readlane((int)y, 63), yh = __builtin_amdgcn_readlane((int)(y >> 32), 63);
+  a = __longlong_as_double(((long long)xh << 32) | (unsigned)xl);
+  b = __longlong_as_double(((long long)yh << 32) | (unsigned)yl);
+}
+
 // The eigensolver workgroups are single wavefronts: a wavefront's LDS
 // accesses execute in program order, so ordering the compiler is enough.
 __device__ __forceinline__ void wsync() {

@@ -327,14 +327,23 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
 
   // Chunk of consecutive points: its active slots own a K x K Gram and
   // eigenvector block and the eigensolve's rotation log.  The register
-  // eigensolve (K <= 128) wants many matrices per launch, so its chunks are
-  // sized by a ~24 GB workspace budget; the LDS path keeps 2 GiB per block.
+  // eigensolve (K <= 128) launches once per (chunk, size bucket) and needs
+  // tens of thousands of matrices per launch to fill the GPU, so its chunks
+  // are sized by a workspace budget of up to 96 GB: 45% of the device memory
+  // free now plus what this handle's per-slot buffers already hold.  The
+  // LDS path keeps 2 GiB per block.
   long long PC;
   if (K <= 128) {
     const double ncol = 1.0 + p.nv3d + p.nv2d;
     const double per_slot = 16.0 * K * K + 8.0 * K * ncol + 16.0 * eigr_rot_capacity(K) +
                             (double)eigr_ix_pitch(K) + 8.0 * eigr_sweep_capacity(K) + 64.0 * K + 8.0;
-    PC = (long long)(24e9 / per_slot) / 256 * 256;  // multiple of 256 (k_bucket_scatter)
+    size_t mfree = 0, mtotal = 0;
+    HIP_TRY(hipMemGetInfo(&mfree, &mtotal));
+    const double held = (double)(h->G.bytes + h->V.bytes + h->yvec.bytes + h->lam.bytes + h->bvec.bytes +
+                                 h->eD.bytes + h->eE.bytes + h->logc.bytes + h->logs.bytes + h->logi.bytes +
+                                 h->logn.bytes + h->logx.bytes + h->logr.bytes);
+    const double budget = std::min(96e9, std::max(1e9, 0.45 * ((double)mfree + held)));
+    PC = (long long)(budget / per_slot) / 256 * 256;  // multiple of 256 (k_bucket_scatter)
   } else {
     PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
     PC = std::min(PC, 65536LL);
