@@ -41,6 +41,13 @@ struct ObsSource {
   }
   __device__ double dep(int l) const { return (double)tb.odep[lobs[o0 + l]]; }
   __device__ void row_of(int io, int K, int m, double& y) const { y = yT[(long long)io * K + m]; }
+  // row l of diag(rloc/rdiag) Y: obs index, weight and innovation
+  __device__ void wrow(int l, int& io, double& w, double& d) const {
+    io = lobs[o0 + l];
+    const T er = tb.oerr[io];
+    w = lrloc[o0 + l] / (double)(T)(er * er);
+    d = (double)tb.odep[io];
+  }
   // row l of B = diag(sqrt(rloc/rdiag)) Y: obs index and sqrt weight
   __device__ void brow(int l, int& io, double& sw) const {
     io = lobs[o0 + l];
@@ -70,6 +77,11 @@ struct CsrSource {
     sw = sqrt((double)rloc[o0 + l] / (double)rdiag[o0 + l]);
   }
   __device__ void row_of(int io, int K, int m, double& y) const { y = (double)hdxf[(long long)io * K + m]; }
+  __device__ void wrow(int l, int& io, double& w, double& d) const {
+    io = (int)(o0 + l);
+    w = (double)rloc[o0 + l] / (double)rdiag[o0 + l];
+    d = (double)depv[o0 + l];
+  }
 };
 
 // KT: padded ensemble tile (64 or 128); each of the 16x16 threads owns a
@@ -256,12 +268,16 @@ constexpr int kMfPanel = 32;  // obs (primal) or members (dual) per LDS panel
 template <int PREC, int OUT, typename Src>
 __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, double* __restrict__ G,
                                                    double* __restrict__ bvec, int dual) {
-  constexpr int TW = OUT / 64;  // tiles per wavefront along each dimension
+  constexpr int TW = OUT / 64;   // tiles per wavefront along each dimension
+  constexpr int LP = OUT + 1;    // odd pitch: row-wise and column-wise panel writes hit distinct banks
+  constexpr int RS = 256 / OUT;  // primal: threads sharing one output column
   // panel: rows = summation index (obs or member), cols = output index
-  __shared__ float As[kMfPanel][OUT + 4];
-  __shared__ float Bs[kMfPanel][OUT + 4];
-  __shared__ double sws[OUT];
+  __shared__ float As[kMfPanel][LP];
+  __shared__ float Bs[kMfPanel][LP];
+  __shared__ double sws[OUT];  // dual: sqrt weight per obs; primal: weight per panel row
+  __shared__ double dps[kMfPanel];
   __shared__ int ios[OUT];
+  __shared__ double bred[256];
   const int n = src.begin(blockIdx.x);
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int r0 = (wv >> 1) * (OUT / 2), c0 = (wv & 1) * (OUT / 2);  // quadrant of this wave
@@ -282,42 +298,57 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
       sws[l] = sw;
       bvec[(long long)blockIdx.x * K + l] = sw * src.dep(l);
     }
-  } else if (t < K) {
-    // b = Rlinv_dYf^T dep in FP64 (member t)
-    double bacc = 0.0;
-    for (int l = 0; l < n; ++l) {
-      double y, rl, rd;
-      src.row(l, K, t, y, rl, rd);
-      bacc = fma(y * rl / rd, src.dep(l), bacc);
-    }
-    bvec[(long long)blockIdx.x * K + t] = bacc;
+    __syncthreads();
   }
-  __syncthreads();
   // tiles of this wave inside the N x N output
   bool live[TW][TW];
 #pragma unroll
   for (int a = 0; a < TW; ++a)
 #pragma unroll
     for (int c = 0; c < TW; ++c) live[a][c] = r0 + 32 * a < N && c0 + 32 * c < N;
+  // primal: b = Y^T diag(rloc/rdiag) dep in FP64, thread t summing column
+  // t % OUT over the panel rows it stages
+  double bpart = 0.0;
   for (int s0 = 0; s0 < S; s0 += kMfPanel) {
     const int ps = min(kMfPanel, S - s0);
-    for (int q = t; q < kMfPanel * OUT; q += 256) {
-      const int row = q / OUT, col = q % OUT;
-      double av = 0.0, bv = 0.0;
-      if (row < ps && col < N) {
-        if (dm) {
+    if (dm) {
+      // B^T panel: lanes walk the members of one obs (contiguous in yT)
+      for (int q = t; q < kMfPanel * OUT; q += 256) {
+        const int row = q & (kMfPanel - 1), col = q / kMfPanel;
+        float v = 0.f;
+        if (row < ps && col < N) {
           double y;
           src.row_of(ios[col], K, s0 + row, y);
-          av = bv = sws[col] * y;
-        } else {
-          double rl, rd;
-          src.row(s0 + row, K, col, bv, rl, rd);
-          av = bv * rl / rd;
+          v = (float)(sws[col] * y);
         }
+        As[row][col] = v;
+        Bs[row][col] = v;
       }
-      // FP32 panel; BF16X3 splits it into hi/lo bf16 at use (16 bits kept)
-      As[row][col] = (float)av;
-      Bs[row][col] = (float)bv;
+    } else {
+      if (t < ps) {
+        int io;
+        double w, d;
+        src.wrow(s0 + t, io, w, d);
+        ios[t] = io;
+        sws[t] = w;
+        dps[t] = d;
+      }
+      __syncthreads();
+      // Y panel: lanes walk the members of one obs
+      for (int q = t; q < kMfPanel * OUT; q += 256) {
+        const int row = q / OUT, col = q % OUT;
+        float av = 0.f, bv = 0.f;
+        if (row < ps && col < N) {
+          double y;
+          src.row_of(ios[row], K, col, y);
+          const double wy = y * sws[row];
+          bpart = fma(wy, dps[row], bpart);
+          av = (float)wy;
+          bv = (float)y;
+        }
+        As[row][col] = av;
+        Bs[row][col] = bv;
+      }
     }
     __syncthreads();
     if (PREC == LETKF_PREC_FP32) {
@@ -364,6 +395,16 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
       }
     }
     __syncthreads();
+  }
+  if (!dm) {
+    bred[t] = bpart;
+    __syncthreads();
+    if (t < K) {
+      double bs = 0.0;
+#pragma unroll
+      for (int r = 0; r < RS; ++r) bs += bred[t + r * OUT];
+      bvec[(long long)blockIdx.x * K + t] = bs;
+    }
   }
   double* Gp = G + (long long)blockIdx.x * K * K;
 #pragma unroll
