@@ -271,6 +271,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   constexpr int TW = OUT / 64;   // tiles per wavefront along each dimension
   constexpr int LP = OUT + 1;    // odd pitch: row-wise and column-wise panel writes hit distinct banks
   constexpr int RS = 256 / OUT;  // primal: threads sharing one output column
+  constexpr int kStage = kMfPanel * OUT / 256;  // panel elements staged per thread (8 or 16)
+  constexpr int kBatch = OUT == 128 ? 8 : 1;    // loads issued together (OUT 64 keeps occupancy 8)
   // panel: rows = summation index (obs or member), cols = output index
   __shared__ float As[kMfPanel][LP];
   __shared__ float Bs[kMfPanel][LP];
@@ -312,17 +314,24 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   for (int s0 = 0; s0 < S; s0 += kMfPanel) {
     const int ps = min(kMfPanel, S - s0);
     if (dm) {
-      // B^T panel: lanes walk the members of one obs (contiguous in yT)
-      for (int q = t; q < kMfPanel * OUT; q += 256) {
-        const int row = q & (kMfPanel - 1), col = q / kMfPanel;
-        float v = 0.f;
-        if (row < ps && col < N) {
-          double y;
-          src.row_of(ios[col], K, s0 + row, y);
-          v = (float)(sws[col] * y);
+      // B^T panel: lanes walk the members of one obs (contiguous in yT);
+      // a batch of kBatch loads is issued before its first store
+#pragma unroll 1
+      for (int u0 = 0; u0 < kStage; u0 += kBatch) {
+        double yv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          const int q = t + 256 * (u0 + u), row = q & (kMfPanel - 1), col = q / kMfPanel;
+          yv[u] = 0.0;
+          if (row < ps && col < N) src.row_of(ios[col], K, s0 + row, yv[u]);
         }
-        As[row][col] = v;
-        Bs[row][col] = v;
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          const int q = t + 256 * (u0 + u), row = q & (kMfPanel - 1), col = q / kMfPanel;
+          const float v = col < N ? (float)(sws[col] * yv[u]) : 0.f;
+          As[row][col] = v;
+          Bs[row][col] = v;
+        }
       }
     } else {
       if (t < ps) {
@@ -334,20 +343,29 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
         dps[t] = d;
       }
       __syncthreads();
-      // Y panel: lanes walk the members of one obs
-      for (int q = t; q < kMfPanel * OUT; q += 256) {
-        const int row = q / OUT, col = q % OUT;
-        float av = 0.f, bv = 0.f;
-        if (row < ps && col < N) {
-          double y;
-          src.row_of(ios[row], K, col, y);
-          const double wy = y * sws[row];
-          bpart = fma(wy, dps[row], bpart);
-          av = (float)wy;
-          bv = (float)y;
+      // Y panel: lanes walk the members of one obs, batches of kBatch loads
+#pragma unroll 1
+      for (int u0 = 0; u0 < kStage; u0 += kBatch) {
+        double yv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          const int q = t + 256 * (u0 + u), row = q / OUT, col = q % OUT;
+          yv[u] = 0.0;
+          if (row < ps && col < N) src.row_of(ios[row], K, col, yv[u]);
         }
-        As[row][col] = av;
-        Bs[row][col] = bv;
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          const int q = t + 256 * (u0 + u), row = q / OUT, col = q % OUT;
+          float av = 0.f, bv = 0.f;
+          if (row < ps && col < N) {
+            const double wy = yv[u] * sws[row];
+            bpart = fma(wy, dps[row], bpart);
+            av = (float)wy;
+            bv = (float)yv[u];
+          }
+          As[row][col] = av;
+          Bs[row][col] = bv;
+        }
       }
     }
     __syncthreads();
