@@ -169,13 +169,17 @@ __global__ void k_gather_bounds(int nb, const long long* pts, const int64_t* lof
 // out[2] sum of n^2 over dual points, out[3] sum of n over the others
 // (Gram flops 2 n^2 K and 2 n K^2), out[4] sum of m^3 over active points,
 // m the eigenproblem size.
-__global__ void k_point_stats(long long P, const int32_t* nloc, int K, int dual, unsigned long long* out) {
-  __shared__ unsigned long long red[5][256];
-  unsigned long long a[5] = {0, 0, 0, 0, 0};
+__global__ void k_point_stats(long long P, long long surf0, const int32_t* nloc, int K, int dual,
+                              unsigned long long* out) {
+  __shared__ unsigned long long red[6][256];
+  unsigned long long a[6] = {0, 0, 0, 0, 0, 0};
   for (long long p = blockIdx.x * 256LL + threadIdx.x; p < P; p += 256LL * gridDim.x) {
     const unsigned long long n = (unsigned long long)nloc[p];
     a[0] = n > a[0] ? n : a[0];
-    if (n == 0) continue;
+    if (n == 0) {
+      a[5] += p >= surf0;  // surface points without local obs
+      continue;
+    }
     const bool d = dual && n < (unsigned long long)K;
     a[1] += d;
     a[2] += d ? n * n : 0;
@@ -183,18 +187,18 @@ __global__ void k_point_stats(long long P, const int32_t* nloc, int K, int dual,
     const unsigned long long m = d ? n : (unsigned long long)K;
     a[4] += m * m * m;
   }
-  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = a[q];
+  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = a[q];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       red[0][threadIdx.x] = max(red[0][threadIdx.x], red[0][threadIdx.x + o]);
-      for (int q = 1; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+      for (int q = 1; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     atomicMax(out, red[0][0]);
-    for (int q = 1; q < 5; ++q) atomicAdd(out + q, red[q][0]);
+    for (int q = 1; q < 6; ++q) atomicAdd(out + q, red[q][0]);
   }
 }
 
@@ -352,19 +356,20 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   const long long nch = (P + PC - 1) / PC;
   std::vector<long long> bpts(nch + 1);
   for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
-  HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 5)));
+  HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 6)));
   long long* d_pts = h->bounds.as<long long>();
   long long* d_out = d_pts + (nch + 1);
   unsigned long long* d_pst = (unsigned long long*)(d_out + 2 * (nch + 1));
-  HIP_TRY(hipMemsetAsync(d_pst, 0, 5 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_point_stats, dim3(1024), dim3(256), 0, s, P, h->nloc.as<int32_t>(), K, dual, d_pst);
+  HIP_TRY(hipMemsetAsync(d_pst, 0, 6 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_point_stats, dim3(1024), dim3(256), 0, s, P, (long long)p.nij1 * (p.nlev - 1),
+                     h->nloc.as<int32_t>(), K, dual, d_pst);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(d_pts, bpts.data(), sizeof(long long) * (nch + 1), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_gather_bounds, dim3((unsigned)((nch + 1 + 255) / 256)), dim3(256), 0, s, (int)(nch + 1),
                      d_pts, h->loff.as<int64_t>(), h->actidx.as<int32_t>(), d_out);
   HIP_TRY(hipGetLastError());
-  std::vector<long long> bv(2 * (nch + 1) + 5);
-  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * (2 * (nch + 1) + 5), hipMemcpyDeviceToHost, s));
+  std::vector<long long> bv(2 * (nch + 1) + 6);
+  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * (2 * (nch + 1) + 6), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   st.max_nobsl = bv[2 * (nch + 1)];
   st.dual_points = p.form == LETKF_FORM_BATCHED ? bv[2 * (nch + 1) + 1] : 0;
@@ -484,7 +489,8 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,
                                 h->lam.as<double>(), h->V.as<double>(), h->bvec.as<double>(), f3, f2, a3, a2, dual,
                                 h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
-                                h->yT.as<double>(), reg_eig ? h->logn.as<int32_t>() : nullptr, s));
+                                h->yT.as<double>(), reg_eig ? h->logn.as<int32_t>() : nullptr,
+                                h->status.as<int>() + 1, s));
     HIP_TRY(mark(e0 + 4));
   }
   HIP_TRY(hipEventRecord(ev1, s));
@@ -518,7 +524,15 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   st.chunks = nch;
   st.gram_flops = 2.0 * K * ((double)bv[2 * (nch + 1) + 2] + (double)K * (double)bv[2 * (nch + 1) + 3]);
   st.eig_flops = 4.0 / 3.0 * (double)bv[2 * (nch + 1) + 4];
-  st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
+  // the tile transform reads and writes the ensemble of the points it
+  // updates: every point on the LDS path, only points without local obs
+  // (and the rare overflowed eigensolves, not counted) on the register path
+  if (reg_eig) {
+    const double noobs = (double)(P - st.active_points), noobs_surf = (double)bv[2 * (nch + 1) + 5];
+    st.transform_bytes = 2.0 * sizeof(T) * K * (noobs * p.nv3d + noobs_surf * p.nv2d);
+  } else {
+    st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
+  }
   h->stats = st;
   return status_to_rc(status);
 }

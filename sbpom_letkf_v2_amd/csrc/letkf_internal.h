@@ -133,7 +133,8 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
 // D/E K, cs 2*eigr_rot_capacity, ix eigr_ix_pitch bytes, hdr
 // 2*eigr_sweep_capacity, count and nrot 1, Y yp >= K*(1+nv3d+nv2d); Q may
 // alias V.  Slots whose rotation log overflowed get lam/V instead
-// (count[slot] == -1) for launch_transform's fallback mode.
+// (count[slot] == -1, and status[1] set) for launch_transform's fallback
+// mode.
 // Events around every eigensolve kernel launch (stage_timing 2): launch u
 // is bracketed by ev[2u], ev[2u+1] and tagged 4*kind + bucket.
 struct EigTimer {
@@ -182,7 +183,8 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb,
                             T* anal2d, int dual, const int64_t* loff,
                             long long lbase, const int32_t* lobs,
                             const double* lrloc, const double* yT,
-                            const int32_t* fallback_count, hipStream_t s);
+                            const int32_t* fallback_count, const int* ovf,
+                            hipStream_t s);
 
 template <typename T>
 hipError_t launch_copy(const T* src, T* dst, size_t n, hipStream_t s);

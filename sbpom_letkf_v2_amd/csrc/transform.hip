@@ -17,14 +17,12 @@
 // LDS with coalesced loads, each wavefront then owns TI/4 points, lane j
 // owning members j and j+64.
 #include "letkf_internal.h"
+#include "wave.h"
 
 namespace letkf {
 
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+// Wavefront sum (DPP butterfly; every lane receives the total).
+__device__ __forceinline__ double wsum(double v) { return wave_sum(v); }
 
 // Orders this wavefront's LDS writes before its later LDS reads; waves of
 // the workgroup work on different points, so no workgroup barrier here.
@@ -48,10 +46,18 @@ struct DualRows {
   const int32_t* lobs;
   const double* lrloc;
   const double* yT;
-  double alpha;  // (K-1)/rho in r_size
+  double alpha;     // (K-1)/rho in r_size
+  const int* ovf;   // register path: nonzero once any eigensolve log overflowed
 };
 
-template <typename T>
+// What one launch of k_transform updates.
+enum TransformMode {
+  TM_ALL = 0,    // every point (the LDS eigensolve path)
+  TM_NOOBS = 1,  // points without local obs only (letkf_core_noobs)
+  TM_OVF = 2     // points whose eigensolve log overflowed (E4 decomposition)
+};
+
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, long long p0, long long p1, int TI,
                                                    const int32_t* __restrict__ nloc,
                                                    const int32_t* __restrict__ actidx, int a0,
@@ -72,32 +78,43 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
   double* z_s = q_s + K;                             // bank A: q[K], z[nvt][K]
   double* y_s = q_s + (size_t)(1 + nvt) * K;         // bank B: [nvt+1][K]
   int* io_s = (int*)(y_s + (size_t)(1 + nvt) * K);   // K obs indices
-  // in fallback mode a tile with no point left to this kernel (no local obs,
-  // or an overflowed eigensolve log) is done before anything is staged
-  // (and only the points left to it are staged)
+  // a tile with no point for this launch is done before anything is
+  // staged (and only its points are staged)
+  if (MODE == TM_OVF && *dr.ovf == 0) return;
   __shared__ int need_s[16];
   if (t < TI) {
     bool need = t < ntile;
-    if (need && dr.fallback_count) {
+    if (need && MODE != TM_ALL) {
       const long long p = pt + t;
-      need = nloc[p] == 0 || dr.fallback_count[actidx[p] - a0] == -1;
+      need = MODE == TM_NOOBS ? nloc[p] == 0 : nloc[p] > 0 && dr.fallback_count[actidx[p] - a0] == -1;
     }
     need_s[t] = need;
   }
   if (!__syncthreads_or(t < TI && need_s[t])) return;
-  // ---- stage the forecast tile (unrolled: several loads in flight) ----
-#pragma unroll 8
-  for (int e = t; e < nvt * K * TI; e += 256) {
-    const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
-    double x = 0.0;
-    if (need_s[tp]) {
-      const long long p = pt + tp;
-      if (v < nv3)
-        x = (double)X3[p + P * (m + (long long)K * v)];
-      else if (p >= surf0)
-        x = (double)X2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))];
+  // ---- stage the forecast tile ----
+  // Thread t keeps point tp = t % TI (TI is 8 or 16) and walks the (variable,
+  // member) rows vm = t / TI, t / TI + 256 / TI, ...; element e = vm TI + tp.
+  const int tpt = t & (TI - 1), vstep = 256 / TI;
+  const bool tneed = need_s[tpt];
+  {
+    int m = (t / TI) % K, v = (t / TI) / K;
+#pragma unroll 4
+    for (int vm = t / TI; vm < nvt * K; vm += vstep) {
+      double x = 0.0;
+      if (tneed) {
+        const long long p = pt + tpt;
+        if (v < nv3)
+          x = (double)X3[p + P * (m + (long long)K * v)];
+        else if (p >= surf0)
+          x = (double)X2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))];
+      }
+      xs[vm * TI + tpt] = x;
+      m += vstep;
+      while (m >= K) {
+        m -= K;
+        ++v;
+      }
     }
-    xs[e] = x;
   }
   __syncthreads();
   const T rho_r = (T)kp.rho;
@@ -116,8 +133,46 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
     const int k = (int)(p / kp.nij1), i = (int)(p - (long long)k * kp.nij1);
     const bool surf = p >= surf0;
     const int nvp = surf ? nvt : nv3;
+    if (!need_s[tp]) continue;  // updated by the fused eigensolve
+    const bool clampable = kp.roff && k >= 1 && !((T)tb.phi1[i] == (T)1);
+    if (MODE == TM_NOOBS) {
+      // letkf_core_noobs: W = sqrt(rho) I, w = 0, then RTPP / RTPS /
+      // round_off as below, one variable at a time
+      for (int v = 0; v < nvp; ++v) {
+        const double x0 = h0 ? xs[((size_t)v * K + m0) * TI + tp] : 0.0;
+        const double x1 = h1 ? xs[((size_t)v * K + m1) * TI + tp] : 0.0;
+        const double fmv = wsum(x0 + x1) * rK;
+        const double d0 = h0 ? x0 - fmv : 0.0, d1 = h1 ? x1 - fmv : 0.0;
+        const double fsv = sqrt(wsum(d0 * d0 + d1 * d1) / (double)(K - 1));
+        const double a0v = fmv + d0 * sqrt_rho, a1v = fmv + d1 * sqrt_rho;
+        const double am = wsum(h0 * a0v + h1 * a1v) * rK;
+        const double da0 = h0 ? a0v - am : 0.0, da1 = h1 ? a1v - am : 0.0;
+        const double asd = sqrt(wsum(da0 * da0 + da1 * da1) / (double)(K - 1));
+        double y0 = a0v, y1 = a1v;
+        if (ap > 0.0) {
+          y0 = am + ap * d0 + (1.0 - ap) * da0;
+          y1 = am + ap * d1 + (1.0 - ap) * da1;
+        }
+        if (as_ > 0.0) {
+          const double f = asd == 0.0 ? 0.0 : as_ * fsv / asd + (1.0 - as_);
+          y0 = am + da0 * f;
+          y1 = am + da1 * f;
+        }
+        if (clampable && v < nv3) {
+          if (v == kp.iv3d_t - 1) {
+            y0 = fmin(fmax(y0, tmin), tmax);
+            y1 = fmin(fmax(y1, tmin), tmax);
+          } else if (v == kp.iv3d_s - 1) {
+            y0 = fmin(fmax(y0, smin), smax);
+            y1 = fmin(fmax(y1, smin), smax);
+          }
+        }
+        if (h0) xs[((size_t)v * K + m0) * TI + tp] = y0;
+        if (h1) xs[((size_t)v * K + m1) * TI + tp] = y1;
+      }
+      continue;
+    }
     const int n = nloc[p];
-    if (dr.fallback_count && n > 0 && dr.fallback_count[actidx[p] - a0] != -1) continue;
     // per-variable forecast mean and perturbations (members m0, m1)
     double dx0[kMaxVars], dx1[kMaxVars], fm[kMaxVars], fs[kMaxVars], ta0[kMaxVars], ta1[kMaxVars];
     _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
@@ -128,7 +183,7 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
       dx1[v] = h1 ? x1 - fm[v] : 0.0;
       fs[v] = sqrt(wsum(dx0[v] * dx0[v] + dx1[v] * dx1[v]) / (double)(K - 1));
     }
-    if (n == 0) {
+    if (MODE == TM_NOOBS || n == 0) {
       _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
         ta0[v] = fm[v] + dx0[v] * sqrt_rho;
         ta1[v] = fm[v] + dx1[v] * sqrt_rho;
@@ -319,7 +374,6 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
       wave_sync();
     }
     // analysis mean / spread / perturbation, RTPP, RTPS, round_off
-    const bool clampable = kp.roff && k >= 1 && !((T)tb.phi1[i] == (T)1);
     _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
       const double am = wsum(h0 * ta0[v] + h1 * ta1[v]) * rK;
       const double da0 = h0 ? ta0[v] - am : 0.0, da1 = h1 ? ta1[v] - am : 0.0;
@@ -349,15 +403,21 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
   }
   __syncthreads();
   // ---- write the analysis tile ----
-  for (int e = t; e < nvt * K * TI; e += 256) {
-    const int tp = e % TI, vm = e / TI, m = vm % K, v = vm / K;
-    if (tp >= ntile) continue;
-    const long long p = pt + tp;
-    if (dr.fallback_count && nloc[p] > 0 && dr.fallback_count[actidx[p] - a0] != -1) continue;
+  // points past the tile, or updated by the fused eigensolve, are skipped
+  if (!tneed) return;
+  int m = (t / TI) % K, v = (t / TI) / K;
+  const long long p = pt + tpt;
+  for (int vm = t / TI; vm < nvt * K; vm += vstep) {
+    const double y = xs[vm * TI + tpt];
     if (v < nv3)
-      A3[p + P * (m + (long long)K * v)] = (T)xs[e];
+      A3[p + P * (m + (long long)K * v)] = (T)y;
     else if (p >= surf0)
-      A2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))] = (T)xs[e];
+      A2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))] = (T)y;
+    m += vstep;
+    while (m >= K) {
+      m -= K;
+      ++v;
+    }
   }
 }
 
@@ -367,34 +427,48 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0
                             const double* V, const double* bvec, const T* fcst3d, const T* fcst2d, T* anal3d,
                             T* anal2d, int dual, const int64_t* loff, long long lbase, const int32_t* lobs,
                             const double* lrloc, const double* yT, const int32_t* fallback_count,
-                            hipStream_t s) {
+                            const int* ovf, hipStream_t s) {
   if (p1 <= p0) return hipSuccess;
   const int nvt = kp.nv3d + kp.nv2d;
   if (nvt > kMaxVars) return hipErrorInvalidValue;
   const int TI = kp.K <= 64 ? 16 : 8;
-  const size_t lds = sizeof(double) * ((size_t)nvt * kp.K * TI +
-                                       4 * (2 * (size_t)(1 + nvt) * kp.K + kp.K / 2 + 1));
-  DualRows dr{dual, fallback_count, loff, lbase, lobs, lrloc, yT, (double)((T)(kp.K - 1) / (T)kp.rho)};
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_transform<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
+  const size_t tile = sizeof(double) * (size_t)nvt * kp.K * TI;
+  const size_t lds = tile + sizeof(double) * 4 * (2 * (size_t)(1 + nvt) * kp.K + kp.K / 2 + 1);
+  DualRows dr{dual, fallback_count, loff, lbase, lobs, lrloc, yT, (double)((T)(kp.K - 1) / (T)kp.rho), ovf};
+  const long long blocks = (p1 - p0 + TI - 1) / TI;
+  auto big = [&](auto kern) -> hipError_t {
+    if (lds > 65536) {
+      hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), lds, s, kp, tb, p0, p1, TI, nloc, actidx, a0, lam,
+                       V, bvec, fcst3d, fcst2d, anal3d, anal2d, dr);
+    return hipGetLastError();
+  };
+  if (!fallback_count) return big(k_transform<T, TM_ALL>);
+  // register path: the points without local obs on the lean variant (LDS
+  // holds the tile only), then the overflowed eigensolves, if any
+  if (tile > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_transform<T, TM_NOOBS>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile);
     if (e != hipSuccess) return e;
   }
-  const long long blocks = (p1 - p0 + TI - 1) / TI;
-  hipLaunchKernelGGL(k_transform<T>, dim3((unsigned)blocks), dim3(256), lds, s, kp, tb, p0, p1, TI, nloc, actidx,
-                     a0, lam, V, bvec, fcst3d, fcst2d, anal3d, anal2d, dr);
-  return hipGetLastError();
+  hipLaunchKernelGGL((k_transform<T, TM_NOOBS>), dim3((unsigned)blocks), dim3(256), tile, s, kp, tb, p0, p1, TI,
+                     nloc, actidx, a0, lam, V, bvec, fcst3d, fcst2d, anal3d, anal2d, dr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return big(k_transform<T, TM_OVF>);
 }
 
 template hipError_t launch_transform<float>(const KParams&, const Tables<float>&, long long, long long,
                                             const int32_t*, const int32_t*, int, const double*, const double*,
                                             const double*, const float*, const float*, float*, float*, int,
                                             const int64_t*, long long, const int32_t*, const double*,
-                                            const double*, const int32_t*, hipStream_t);
+                                            const double*, const int32_t*, const int*, hipStream_t);
 template hipError_t launch_transform<double>(const KParams&, const Tables<double>&, long long, long long,
                                              const int32_t*, const int32_t*, int, const double*, const double*,
                                              const double*, const double*, const double*, double*, double*,
                                              int, const int64_t*, long long, const int32_t*, const double*,
-                                             const double*, const int32_t*, hipStream_t);
+                                             const double*, const int32_t*, const int*, hipStream_t);
 
 }  // namespace letkf
