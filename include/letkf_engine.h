@@ -122,6 +122,9 @@ typedef struct letkf_stats {
                            /* vectors, 3 rotation replay, 4 Q and the    */
                            /* update; bucket 0..4 = size <= 16, 32, 48,  */
                            /* 64, 128                                    */
+  int64_t fallback_slots;  /* K <= 128: points whose rotation log       */
+                           /* overflowed, solved by the LDS eigensolver */
+                           /* and the full tile transform instead       */
 } letkf_stats_t;
 
 typedef struct letkf_handle letkf_handle_t;

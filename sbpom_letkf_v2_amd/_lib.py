@@ -53,7 +53,8 @@ class Stats(C.Structure):
         (n, C.c_double) for n in (
             "ms_total", "ms_search", "ms_gram", "ms_eig", "ms_transform",
             "gram_flops", "transform_bytes")] + [("chunks", C.c_int64), ("eig_flops", C.c_double),
-                                                 ("ms_eigk", C.c_double * 25), ("n_eigk", C.c_int64 * 25)]
+                                                 ("ms_eigk", C.c_double * 25), ("n_eigk", C.c_int64 * 25),
+                                                 ("fallback_slots", C.c_int64)]
 
 
 class EngineError(RuntimeError):

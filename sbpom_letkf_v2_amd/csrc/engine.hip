@@ -494,9 +494,11 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     HIP_TRY(mark(e0 + 4));
   }
   HIP_TRY(hipEventRecord(ev1, s));
-  int status = 0;
-  HIP_TRY(hipMemcpyAsync(&status, h->status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  int stat4[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(stat4, h->status.ptr, sizeof(stat4), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  const int status = stat4[0];
+  st.fallback_slots = stat4[2];
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
   (void)hipEventDestroy(ev0);

@@ -218,6 +218,24 @@ def test_aoei_gpu_vs_oracle():
     assert np.array_equal(te.cpu().numpy(), po.aoei(e, d, s))
 
 
+def test_rotation_log_overflow_fallback(monkeypatch):
+    """Slots whose QL rotation log overflows are solved whole by the LDS
+    eigensolver and updated by the full tile transform.  A capped log
+    (LETKF_ROT_LOG_CAP) sends the larger slots down that path while the
+    small ones stay on the register path; both must meet the FP64 bar."""
+    monkeypatch.setenv("LETKF_ROT_LOG_CAP", "300")
+    _say("case rotation-log overflow fallback K=32")
+    pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=32, nobs=800, seed=71, alpha_rtpp=0.7,
+                            alpha_rtps=0.3, land_frac=0.1, real_kind=8, sigma_h_cells=1.0, sigma_v=1.0)
+    a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched")
+    _say(f"  fallback slots {st['fallback_slots']} of {st['active_points']}")
+    assert 0 < st["fallback_slots"] < st["active_points"]
+    e3 = per_var_err(a3, a3o, axis_var=3)
+    e2 = per_var_err(a2, a2o, axis_var=2)
+    _say(f"  err3={e3} err2={e2}")
+    assert max(e3) <= 1e-10 and max(e2) <= 1e-10
+
+
 @pytest.mark.parametrize("precision,form,K,nobs", [("fp32", "batched", 48, 900), ("bf16x3", "batched", 48, 900),
                                                    ("fp32", "plain", 48, 900), ("fp32", "batched", 100, 1800),
                                                    ("bf16x3", "batched", 128, 2000), ("fp32", "plain", 100, 1800)])
