@@ -50,7 +50,40 @@ def parse():
                     help="differentiable workload (default c3): time forward + hand-written adjoint and "
                          "check it by central finite differences")
     ap.add_argument("--fd-coords", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only an elementwise kernel that reads and writes 1 GiB (three launches), the "
+                         "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 28  # 2^28 float32 = 1 GiB read and 1 GiB written per launch
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0001, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "elementwise mul", "read_bytes_per_launch": 4 * n,
+                      "write_bytes_per_launch": 4 * n, "launches": 3}))
+
+
+def profile_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` (a kernel family) from the committed
+    rocprofv3 summary of this workload: the FETCH_SIZE and WRITE_SIZE passes,
+    each scaled by the counter's reading for the 1 GiB copy kernel."""
+    import glob
+    pat = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                       f"kernels_{args.workload}_*_{args.form}_{args.precision}.md")
+    for path in sorted(glob.glob(pat)):
+        for line in open(path):
+            cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+            if len(cells) == 6 and cells[0] == kernel and cells[1] == "traffic":
+                try:
+                    return float(cells[5])
+                except ValueError:
+                    return None
+    return None
 
 
 def run_adjoint(args):
@@ -256,6 +289,9 @@ def main():
     if args.adjoint:
         run_adjoint(args)
         return
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import numpy as np
     import torch
     import pyoracle as po
@@ -299,6 +335,9 @@ def main():
         return ms, n
 
     tri_ms, tri_n = family("k_eigr_tridiag")
+    ql_ms, ql_n = family("k_eigr_ql")
+    vr_ms, vr_n = family("k_eigr_vrot")
+    rot = float(st.get("rotations", 0))
     gram_kernel = "k_gram_fp64" if args.precision == "fp64" or K > 128 else "k_gram_mfma"
     kernels = {
         # Gram: 2 n K^2 (primal) or 2 n^2 K (dual) flops per point, one launch per chunk
@@ -307,17 +346,23 @@ def main():
         # Householder tridiagonalisation: (4/3) m^3 flops per point, FP64
         "k_eigr_tridiag": dict(ms=tri_ms, launches=tri_n, work=st["eig_flops"], unit="TFLOP/s",
                                bound="mfma", peak=PEAK_FP64_TFLOPS),
+        # implicit QL: ~30 FP64 flops per logged rotation (Givens pair, refined rsqrt, shift update)
+        "k_eigr_ql": dict(ms=ql_ms, launches=ql_n, work=30.0 * rot, unit="TFLOP/s",
+                          bound="mfma", peak=PEAK_FP64_TFLOPS),
+        # rotation replay: every logged rotation on the 1+nv3d+nv2d vectors, forward and back, 6 flops each
+        "k_eigr_vrot": dict(ms=vr_ms, launches=vr_n, work=12.0 * (1 + nv3 + nv2) * rot, unit="TFLOP/s",
+                            bound="mfma", peak=PEAK_FP64_TFLOPS),
         # tile update (points without local obs and overflowed eigensolves): read + write of the ensemble
         "k_transform": dict(ms=stage["ms_transform"], launches=nch, work=st["transform_bytes"],
                             unit="GB/s", bound="hbm", peak=PEAK_HBM_GBS),
     }
-    kernels = {k: v for k, v in kernels.items() if v["launches"] and v["ms"] > 0}
+    kernels = {k: v for k, v in kernels.items() if v["launches"] and v["ms"] > 0 and v["work"] > 0}
     dom_name = max(kernels, key=lambda k: kernels[k]["ms"])
     dom = kernels[dom_name]
     scale = 1e12 if dom["unit"] == "TFLOP/s" else 1e9
     achieved = dom["work"] / (dom["ms"] * 1e-3) / scale
     roofline = {"bound": dom["bound"], "achieved": achieved, "peak": dom["peak"], "unit": dom["unit"],
-                "frac": achieved / dom["peak"], "traffic": None, "kernel": dom_name,
+                "frac": achieved / dom["peak"], "traffic": profile_traffic(args, dom_name), "kernel": dom_name,
                 "launches_per_step": dom["launches"], "avg_launch_ms": dom["ms"] / dom["launches"],
                 "work_per_launch": dom["work"] / dom["launches"]}
     kernel_roofline = {}

@@ -122,6 +122,9 @@ typedef struct letkf_stats {
                            /* vectors, 3 rotation replay, 4 Q and the    */
                            /* update; bucket 0..4 = size <= 16, 32, 48,  */
                            /* 64, 128                                    */
+  int64_t rotations;       /* stage_timing 2, K <= 128: QL rotations     */
+                           /* logged over all slots (each replayed on   */
+                           /* 1+nv3d+nv2d vectors, twice)               */
   int64_t fallback_slots;  /* K <= 128: points whose rotation log       */
                            /* overflowed, solved by the LDS eigensolver */
                            /* and the full tile transform instead       */

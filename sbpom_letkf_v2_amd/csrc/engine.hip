@@ -433,12 +433,14 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
 
   if (timing) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
   // per-kernel marks of the register eigensolve: 5 kernels per bucket launch
-  EigTimer ktm{nullptr, nullptr, 0, 0};
+  EigTimer ktm{nullptr, nullptr, 0, 0, nullptr};
   if (p.stage_timing >= 2 && reg_eig) {
     const int cap = (int)(5 * kNQ * nch);
     HIP_TRY(h->ensure_kevents(2 * (size_t)cap));
     h->ktags.assign(cap, 0);
-    ktm = EigTimer{h->kevents.data(), h->ktags.data(), cap, 0};
+    HIP_TRY(h->rotsum.ensure(sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(h->rotsum.ptr, 0, sizeof(unsigned long long), s));
+    ktm = EigTimer{h->kevents.data(), h->ktags.data(), cap, 0, h->rotsum.as<unsigned long long>()};
   }
   for (long long c = 0; c < nch; ++c) {
     const size_t e0 = 2 + 5 * (size_t)c;
@@ -499,6 +501,11 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   HIP_TRY(hipStreamSynchronize(s));
   const int status = stat4[0];
   st.fallback_slots = stat4[2];
+  if (ktm.rot) {
+    unsigned long long nrot_total = 0;
+    HIP_TRY(hipMemcpy(&nrot_total, ktm.rot, sizeof(nrot_total), hipMemcpyDeviceToHost));
+    st.rotations = (int64_t)nrot_total;
+  }
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
   (void)hipEventDestroy(ev0);

@@ -54,6 +54,7 @@ struct letkf_handle {
   DevBuf nloc, loff, actidx, act_p, scan_tmp, yT, lobs, lrloc, G, lam, V, bvec, status, bounds;
   DevBuf nrow, bin_tmp;
   DevBuf eD, eE, logc, logs, logi, logn, logx, logr, yvec;  // eigensolve workspaces
+  DevBuf rotsum;  // stage_timing 2: logged QL rotations, summed over slots
   DevBuf adj_part;  // adjoint: per-point (dL/dalpha, dL/drho)
   DevBuf bhist, bcur, blist;              // per-chunk size buckets of active slots
   // staging for the host-pointer entry point
@@ -82,7 +83,7 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &adj_part, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &adj_part, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();

@@ -141,6 +141,7 @@ struct EigTimer {
   hipEvent_t* ev;
   int* tag;
   int cap, used;
+  unsigned long long* rot;  // device counter: logged rotations of every slot (may be null)
 };
 
 long long eigr_rot_capacity(int K);
