@@ -152,6 +152,15 @@ static KParams make_kparams(const letkf_params_t& p, int nobs) {
   kp.tmax = p.tmax;
   kp.smin = p.smin;
   kp.smax = p.smax;
+  kp.cos_in = 2.0;  // no shortcut unless the cut-off angle is well inside (0, pi)
+  kp.cos_out = -2.0;
+  if (p.dist_zero > 0.0 && p.re > 0.0) {
+    const double th = p.dist_zero / p.re, m = p.real_kind == 4 ? 1e-4 : 1e-9;
+    if (th * (1.0 + m) < 3.0) {
+      kp.cos_in = cos(th * (1.0 - m));
+      kp.cos_out = cos(th * (1.0 + m));
+    }
+  }
   return kp;
 }
 
@@ -306,6 +315,9 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   tb.olev = (const T*)o->lev;
   tb.oerr = (const T*)o->err;
   tb.odep = (const T*)o->dep;
+  HIP_TRY(h->otrig.ensure(sizeof(double2) * (size_t)(o->nobs + 1)));
+  tb.osc = h->otrig.as<double2>();
+  HIP_TRY(launch_obs_trig<T>(o->nobs, p.pi, tb.olat, h->otrig.as<double2>(), s));
 
   HIP_TRY(h->nloc.ensure(sizeof(int32_t) * (P + 1)));
   HIP_TRY(h->loff.ensure(sizeof(int64_t) * (P + 1)));
@@ -580,6 +592,9 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   tb.olev = (const double*)o->lev;
   tb.oerr = (const double*)o->err;
   tb.odep = (const double*)o->dep;
+  HIP_TRY(h->otrig.ensure(sizeof(double2) * (size_t)(o->nobs + 1)));
+  tb.osc = h->otrig.as<double2>();
+  HIP_TRY(launch_obs_trig<double>(o->nobs, p.pi, tb.olat, h->otrig.as<double2>(), s));
   HIP_TRY(h->nloc.ensure(sizeof(int32_t) * (P + 1)));
   HIP_TRY(h->loff.ensure(sizeof(int64_t) * (P + 1)));
   HIP_TRY(h->actidx.ensure(sizeof(int32_t) * (P + 1)));

@@ -55,6 +55,7 @@ struct letkf_handle {
   DevBuf nrow, bin_tmp;
   DevBuf eD, eE, logc, logs, logi, logn, logx, logr, yvec;  // eigensolve workspaces
   DevBuf rotsum;  // stage_timing 2: logged QL rotations, summed over slots
+  DevBuf otrig;   // (sin, cos) of every obs latitude
   DevBuf adj_part;  // adjoint: per-point (dL/dalpha, dL/drho)
   DevBuf bhist, bcur, blist;              // per-chunk size buckets of active slots
   // staging for the host-pointer entry point
@@ -83,7 +84,7 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &adj_part, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &adj_part, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();

@@ -15,6 +15,10 @@ struct KParams {
   long long P;  // nij1 * nlev
   double pi, re, rho, alpha_rtpp, alpha_rtps, sigma_obs, sigma_obsv;
   double dist_zero, dist_zerov, tmin, tmax, smin, smax;
+  // cos of the great-circle angle dist_zero/re widened by a relative margin
+  // far above the rounding of re*acos in r_size: cos d >= cos_in is inside
+  // the cut-off and cos d <= cos_out beyond it without evaluating acos
+  double cos_in, cos_out;
 };
 
 // Local grid + sorted obs tables, typed by the caller's REAL.
@@ -23,6 +27,7 @@ struct Tables {
   const T *lon1, *lat1, *ri1, *rj1, *dlon1, *dlat1, *phi1, *depth1;
   const int32_t *nobsgrd, *elm;
   const T *olon, *olat, *olev, *oerr, *odep;
+  const double2* osc;  // per obs (sin, cos) of its latitude in radians, as distance_2p forms them
 };
 
 // Per-analysis device arrays over all P points.
@@ -44,6 +49,8 @@ enum : int {
 constexpr int kMaxK = 128;  // largest ensemble handled on chip
 
 // ---- launchers (each defined in the .hip that holds its kernel) ----
+template <typename T>
+hipError_t launch_obs_trig(int nobs, double pi, const T* olat, double2* osc, hipStream_t s);
 template <typename T>
 hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb,
                                int32_t* nloc, hipStream_t s);

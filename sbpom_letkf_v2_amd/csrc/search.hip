@@ -23,7 +23,9 @@ template <typename T> __device__ __forceinline__ T abs_r(T x) { return x < (T)0 
 
 // Walks the search box of point (i,k) in the reference's order and calls
 // visit(io, dist, dlev) for each accepted obs io (0-based).
-template <typename T, typename Visit>
+// NEED_DIST false (count only): an obs whose cos d is clearly inside the
+// cut-off is accepted without acos; dist is then not formed.
+template <typename T, bool NEED_DIST, typename Visit>
 __device__ __forceinline__ int search_point(const KParams& kp, const Tables<T>& tb,
                                             int i, int k, Visit&& visit) {
 #pragma clang fp contract(off)
@@ -64,15 +66,19 @@ __device__ __forceinline__ int search_point(const KParams& kp, const Tables<T>& 
       else
         dlev = (T)0;
       if (dzv < dlev) continue;
-      const double aa = (double)(T)(pi * tb.olat[io]) / 180.0;
+      const double2 sca = tb.osc[io];  // sin(aa), cos(aa) of this obs (k_obs_trig)
       const double cc = (double)(T)(pi * (T)(lonb - tb.olon[io])) / 180.0;
-      const double prod1 = sin(aa) * sinb;
-      const double prod2 = cos(aa) * cosb;
+      const double prod1 = sca.x * sinb;
+      const double prod2 = sca.y * cosb;
       T cosd = (T)(prod1 + prod2 * cos(cc));
       if ((double)cosd > 1.0) cosd = (T)1;
       if ((double)cosd < -1.0) cosd = (T)-1;
-      const T dist = re * acos_r<T>(cosd);
-      if (dz < dist) continue;
+      if ((double)cosd <= kp.cos_out) continue;  // beyond dist_zero whatever acos rounds to
+      T dist = (T)0;
+      if (NEED_DIST || (double)cosd < kp.cos_in) {
+        dist = re * acos_r<T>(cosd);
+        if (dz < dist) continue;
+      }
       visit(count, io, dist, dlev);
       ++count;
     }
@@ -85,7 +91,7 @@ __global__ void __launch_bounds__(256) k_search_count(KParams kp, Tables<T> tb, 
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < kp.P; p += stride) {
     const int k = (int)(p / kp.nij1), i = (int)(p - (long long)k * kp.nij1);
-    nloc[p] = search_point<T>(kp, tb, i, k, [](int, int, T, T) {});
+    nloc[p] = search_point<T, false>(kp, tb, i, k, [](int, int, T, T) {});
   }
 }
 
@@ -103,7 +109,7 @@ __global__ void __launch_bounds__(256) k_search_fill(KParams kp, Tables<T> tb, l
     const long long o = loff[p] - base;
     if (loff[p + 1] == loff[p]) continue;
     const int k = (int)(p / kp.nij1), i = (int)(p - (long long)k * kp.nij1);
-    search_point<T>(kp, tb, i, k, [&](int l, int io, T dist, T dlev) {
+    search_point<T, true>(kp, tb, i, k, [&](int l, int io, T dist, T dlev) {
 #pragma clang fp contract(off)
       T q = (dist / so) * (dist / so);
       if (vloc) q = q + (dlev / sv) * (dlev / sv);
@@ -112,6 +118,28 @@ __global__ void __launch_bounds__(256) k_search_fill(KParams kp, Tables<T> tb, l
     });
   }
 }
+
+// (sin, cos) of every obs latitude in radians, formed as distance_2p forms
+// them (common.f90:79-101): aa = real(pi*lat, r_size)/180 in FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_obs_trig(int nobs, double pi_d, const T* __restrict__ olat,
+                                                  double2* __restrict__ osc) {
+#pragma clang fp contract(off)
+  const int io = blockIdx.x * 256 + threadIdx.x;
+  if (io >= nobs) return;
+  const T pi = (T)pi_d;
+  const double aa = (double)(T)(pi * olat[io]) / 180.0;
+  osc[io] = make_double2(sin(aa), cos(aa));
+}
+
+template <typename T>
+hipError_t launch_obs_trig(int nobs, double pi, const T* olat, double2* osc, hipStream_t s) {
+  if (nobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_obs_trig<T>, dim3((unsigned)((nobs + 255) / 256)), dim3(256), 0, s, nobs, pi, olat, osc);
+  return hipGetLastError();
+}
+template hipError_t launch_obs_trig<float>(int, double, const float*, double2*, hipStream_t);
+template hipError_t launch_obs_trig<double>(int, double, const double*, double2*, hipStream_t);
 
 template <typename T>
 hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb, int32_t* nloc, hipStream_t s) {
