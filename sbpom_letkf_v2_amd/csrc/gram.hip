@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, 
 #pragma unroll
       for (int y = 0; y < TT; ++y) {
         const int c = tc * TT + y;
-        if (c < n) Mp[r * n + c] = acc[x][y];
+        if (c < n && r <= c) Mp[r * n + c] = acc[x][y];  // upper triangle: all its readers use
       }
     }
     if (t == 0 && nrow) nrow[blockIdx.x] = n;
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, 
 #pragma unroll
     for (int y = 0; y < TT; ++y) {
       const int c = tc * TT + y;
-      if (c < K) Gp[r * K + c] = acc[x][y] + (r == c ? diag : 0.0);
+      if (c < K && r <= c) Gp[r * K + c] = acc[x][y] + (r == c ? diag : 0.0);
     }
   }
   if (t < K) bvec[(long long)blockIdx.x * K + t] = bacc;
@@ -433,7 +433,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = r0 + 32 * a + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        if (row < N && col < N) Gp[row * N + col] = (double)acc[a][c][r] + (!dm && row == col ? diag : 0.0);
+        if (row < N && col < N && row <= col)  // upper triangle: all its readers use
+          Gp[row * N + col] = (double)acc[a][c][r] + (!dm && row == col ? diag : 0.0);
       }
     }
 }

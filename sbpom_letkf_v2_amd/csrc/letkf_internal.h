@@ -70,6 +70,8 @@ hipError_t launch_transpose_obs(int nobs, int K, const T* hdxf, double* yT,
 
 // Gram G = Y^T diag(rloc/rdiag) Y + (K-1)/rho I and b = Y^T diag(rloc/rdiag) d
 // for active slots [a0, a0+na) -> G point-major [na][K][K], bvec [na][K].
+// Only the upper triangle (with the diagonal) of each G is written: the
+// eigensolvers read no other entry.
 template <typename T>
 hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
                             const double* yT, const int32_t* act_p,
@@ -80,7 +82,7 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
 
 // The same products with the Gram on MFMA in FP32 (prec LETKF_PREC_FP32)
 // or from BF16 hi/lo splits (LETKF_PREC_BF16X3), FP32 accumulation, b in
-// FP64; K <= 64.
+// FP64; K <= 128.
 template <typename T>
 hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb,
                              const double* yT, const int32_t* act_p,
