@@ -182,6 +182,41 @@ int letkf_bin_obs(letkf_handle_t* h, int32_t nlon, int32_t nlat,
 int letkf_aoei(letkf_handle_t* h, int32_t nobs, void* err, const void* dep,
                const void* hxfsprd);
 
+/* Grid of the observation operator: the global model grid and the element
+ * constants Trans_XtoY reads from common_setting (common_setting.f90:30,
+ * 42-60, 155-161).  Fields are column-major in the reference's layouts:
+ * lon, lat, fsm (nlon,nlat); depth (nlon,nlat,nlev), negative metres,
+ * k = nlev the surface, k = 1 a dummy layer.  Variable slots are 1-based.
+ * All real arrays have the element type of real_kind (4 or 8). */
+typedef struct letkf_xtoy_grid {
+  int32_t nlon, nlat, nlev, nv3d, nv2d;
+  int32_t real_kind;
+  int32_t id_u_obs, id_v_obs, id_t_obs, id_s_obs, id_z_obs; /* 2819 2820 3073 3332 2567 */
+  int32_t iv3d_u, iv3d_v, iv3d_t, iv3d_s, iv2d_z;
+  double hnosea, undef;                                     /* land value of fsm; 999 */
+  const void* lon;
+  const void* lat;
+  const void* depth;
+  const void* fsm;
+} letkf_xtoy_grid_t;
+
+/* H(x) of every member on DEVICE arrays: Trans_XtoY (common.f90:391-500)
+ * called by set_obs once per member (common_io.f90:829-831), batched.
+ * elm/idx/idy/lono/lato/levo describe nobs obs (idx, idy 1-based cells);
+ * v3d (nlon,nlat,nlev,nv3d,nmem) and v2d (nlon,nlat,nv2d,nmem) hold the
+ * members; hx (nobs,nmem) receives H(x_m), undef where the reference gives
+ * undef; qc (nobs, may be NULL) receives 0 where any member is undef (the
+ * tmpqc rule of common_io.f90:833-839), 1 elsewhere.  Requires nlev >= 6
+ * (the Akima stencil).  Runs on `stream` (NULL: the default stream) and
+ * returns when done.  No counterpart entry point in the reference, which
+ * loops over members on the host. */
+int letkf_trans_xtoy_device(const letkf_xtoy_grid_t* g, int32_t nobs,
+                            const int32_t* elm, const int32_t* idx,
+                            const int32_t* idy, const void* lono,
+                            const void* lato, const void* levo, int32_t nmem,
+                            const void* v3d, const void* v2d, void* hx,
+                            int32_t* qc, void* stream);
+
 /* Adjoint of letkf_da_main_device (no counterpart in the reference; for
  * gradient-based uses of the analysis: observation impact, tuning of the
  * inflation and of the observation errors, torch.autograd).  DEVICE arrays;

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:2dc0eeab1da00241
+﻿!mod$ v1 sum:69b4082244a92dd2
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ 2dcd5c82c7cba8f9 n common
 !need$ 7d9124a989da7208 n common_letkf
@@ -9,8 +9,31 @@ use common_setting,only:r_dble
 use common_setting,only:nmem
 use common_setting,only:pi
 use common_setting,only:re
+use common_setting,only:nlon
+use common_setting,only:nlat
+use common_setting,only:nlev
+use common_setting,only:nv3d
+use common_setting,only:nv2d
+use common_setting,only:lon
+use common_setting,only:lat
+use common_setting,only:depth
+use common_setting,only:fsm
+use common_setting,only:undef
+use common_setting,only:hnosea
+use common_setting,only:id_u_obs
+use common_setting,only:id_v_obs
+use common_setting,only:id_t_obs
+use common_setting,only:id_s_obs
+use common_setting,only:id_z_obs
+use common_setting,only:iv3d_u
+use common_setting,only:iv3d_v
+use common_setting,only:iv3d_t
+use common_setting,only:iv3d_s
+use common_setting,only:iv2d_z
 use common,only:distance_2p
 use common,only:mtx_eigen
+use common,only:akima_spline
+use common,only:trans_xtoy
 use common_letkf,only:letkf_core
 use,intrinsic::iso_c_binding,only:c_associated
 use,intrinsic::iso_c_binding,only:c_funloc
@@ -115,6 +138,33 @@ real(4),value::lata
 real(4),value::lonb
 real(4),value::latb
 real(4),intent(out)::dist
+end
+subroutine ref_trans_xtoy(n,elm,idx,idy,lono,lato,levo,v3d,v2d,lon_in,lat_in,depth_in,fsm_in,hx) bind(c,name="ref_trans_xtoy")
+integer(4),value::n
+integer(4),intent(in)::elm(1_8:int(n,kind=8))
+integer(4),intent(in)::idx(1_8:int(n,kind=8))
+integer(4),intent(in)::idy(1_8:int(n,kind=8))
+real(4),intent(in)::lono(1_8:int(n,kind=8))
+real(4),intent(in)::lato(1_8:int(n,kind=8))
+real(4),intent(in)::levo(1_8:int(n,kind=8))
+real(4),intent(in)::v3d(1_8:1442_8,1_8:562_8,1_8:75_8,1_8:4_8)
+real(4),intent(in)::v2d(1_8:1442_8,1_8:562_8,1_8:3_8)
+real(4),intent(in)::lon_in(1_8:1442_8,1_8:562_8)
+real(4),intent(in)::lat_in(1_8:1442_8,1_8:562_8)
+real(4),intent(in)::depth_in(1_8:1442_8,1_8:562_8,1_8:75_8)
+real(4),intent(in)::fsm_in(1_8:1442_8,1_8:562_8)
+real(4),intent(out)::hx(1_8:int(n,kind=8))
+end
+subroutine ref_akima(ndim,x,y,x5,y5) bind(c,name="ref_akima")
+integer(4),value::ndim
+real(4),intent(in)::x(1_8:int(ndim,kind=8))
+real(4),intent(in)::y(1_8:int(ndim,kind=8))
+real(4),value::x5
+real(4),intent(out)::y5
+end
+subroutine ref_xtoy_consts(dims,vals) bind(c,name="ref_xtoy_consts")
+integer(4),intent(out)::dims(1_8:15_8)
+real(4),intent(out)::vals(1_8:2_8)
 end
 subroutine ref_constants(pi_out,re_out) bind(c,name="ref_constants")
 real(4),intent(out)::pi_out

@@ -47,6 +47,18 @@ typedef struct oracle_params {
   const int32_t *nobsgrd, *elm;                                                \
   const REAL *olon, *olat, *olev, *oerr, *odep, *ohdxf;
 
+/* Grid and element constants of the observation operator Trans_XtoY
+ * (common.f90:391-500; common_setting.f90:30,42-60).  Variable slots are
+ * 1-based.  Field layouts: lon, lat, fsm (nlon,nlat); depth (nlon,nlat,nlev)
+ * negative with k = nlev the surface and k = 1 a dummy layer; v3d
+ * (nlon,nlat,nlev,nv3d); v2d (nlon,nlat,nv2d). */
+typedef struct oracle_xtoy {
+  int32_t nlon, nlat, nlev, nv3d, nv2d;
+  int32_t id_u_obs, id_v_obs, id_t_obs, id_s_obs, id_z_obs;
+  int32_t iv3d_u, iv3d_v, iv3d_t, iv3d_s, iv2d_z;
+  double hnosea, undef;
+} oracle_xtoy_t;
+
 typedef struct oracle_tables_f64 { ORACLE_TABLES(double) } oracle_tables_f64_t;
 typedef struct oracle_tables_f32 { ORACLE_TABLES(float) } oracle_tables_f32_t;
 
@@ -78,7 +90,16 @@ typedef struct oracle_tables_f32 { ORACLE_TABLES(float) } oracle_tables_f32_t;
   int oracle_bin_obs##SUF(int nlon, int nlat, const REAL* lon,                 \
                           const REAL* lat, int nobs, const REAL* olon,         \
                           const REAL* olat, int32_t* idx, int32_t* idy,        \
-                          int32_t* perm, int32_t* nobsgrd);
+                          int32_t* perm, int32_t* nobsgrd);                \
+  REAL oracle_akima##SUF(int ndim, const REAL* x, const REAL* y, REAL x5,      \
+                         REAL undef);                                          \
+  void oracle_trans_xtoy##SUF(const oracle_xtoy_t* g, int nobs,                \
+                              const int32_t* elm, const int32_t* idx,          \
+                              const int32_t* idy, const REAL* lono,            \
+                              const REAL* lato, const REAL* levo,              \
+                              const REAL* lon, const REAL* lat,                \
+                              const REAL* depth, const REAL* fsm,              \
+                              const REAL* v3d, const REAL* v2d, REAL* hx);
 
 ORACLE_DECL(_f64, double)
 ORACLE_DECL(_f32, float)

@@ -49,6 +49,14 @@ def _tables_struct(ct):
     return type("Tables", (C.Structure,), {"_fields_": fields})
 
 
+class OracleXtoY(C.Structure):
+    """oracle_xtoy_t: grid and element constants of Trans_XtoY."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "nlon", "nlat", "nlev", "nv3d", "nv2d", "id_u_obs", "id_v_obs", "id_t_obs", "id_s_obs",
+        "id_z_obs", "iv3d_u", "iv3d_v", "iv3d_t", "iv3d_s", "iv2d_z")] + [
+        ("hnosea", C.c_double), ("undef", C.c_double)]
+
+
 Tables64 = _tables_struct(C.c_double)
 Tables32 = _tables_struct(C.c_float)
 
@@ -95,6 +103,12 @@ def lib():
             f = getattr(_lib, "oracle_bin_obs" + suf)
             f.restype = C.c_int
             f.argtypes = [C.c_int, C.c_int, P, P, C.c_int, P, P, IP, IP, IP, IP]
+            f = getattr(_lib, "oracle_akima" + suf)
+            f.restype = ct
+            f.argtypes = [C.c_int, P, P, ct, ct]
+            f = getattr(_lib, "oracle_trans_xtoy" + suf)
+            f.restype = None
+            f.argtypes = [C.POINTER(OracleXtoY), C.c_int, IP, IP, IP, P, P, P, P, P, P, P, P, P, P]
     return _lib
 
 
@@ -122,6 +136,13 @@ def ref():
         _ref.ref_distance.restype = None
         _ref.ref_constants.argtypes = [F, F]
         _ref.ref_constants.restype = None
+        I = C.POINTER(C.c_int32)
+        _ref.ref_trans_xtoy.argtypes = [C.c_int, I, I, I, F, F, F, F, F, F, F, F, F, F]
+        _ref.ref_trans_xtoy.restype = None
+        _ref.ref_akima.argtypes = [C.c_int, F, F, C.c_float, F]
+        _ref.ref_akima.restype = None
+        _ref.ref_xtoy_consts.argtypes = [I, F]
+        _ref.ref_xtoy_consts.restype = None
     return _ref
 
 
@@ -338,3 +359,37 @@ def bin_obs(lon2d, lat2d, olon, olat, dtype=np.float64):
         _ptr(olat, ct), idx.ctypes.data_as(IP), idy.ctypes.data_as(IP),
         perm.ctypes.data_as(IP), grd.ctypes.data_as(IP))
     return idx, idy, perm[:s], grd
+
+
+# ---------------------------------------------------------- H(x) operator
+
+
+def akima(x, y, x5, undef=999.0, dtype=np.float64):
+    """akima_spline (common.f90:276-385) at one target."""
+    dt = np.dtype(dtype)
+    ct = _ct(dt)
+    x = np.ascontiguousarray(x, dt)
+    y = np.ascontiguousarray(y, dt)
+    f = getattr(lib(), "oracle_akima" + _suffix(dt))
+    return float(f(len(x), _ptr(x, ct), _ptr(y, ct), ct(float(x5)), ct(float(undef))))
+
+
+def trans_xtoy(g: OracleXtoY, elm, idx, idy, lono, lato, levo, lon, lat, depth, fsm, v3d, v2d,
+               dtype=np.float64):
+    """Trans_XtoY (common.f90:391-500) of one state; arrays in the Fortran
+    layouts (numpy arrays indexed [i, j, ...] in column-major order)."""
+    dt = np.dtype(dtype)
+    ct = _ct(dt)
+    n = len(elm)
+    I = lambda a: np.ascontiguousarray(a, np.int32)
+    elm, idx, idy = I(elm), I(idx), I(idy)
+    lono, lato, levo = (np.ascontiguousarray(a, dt) for a in (lono, lato, levo))
+    lon, lat, depth, fsm, v3d, v2d = (_f(a, dt) for a in (lon, lat, depth, fsm, v3d, v2d))
+    hx = np.empty(n, dt)
+    IP = C.POINTER(C.c_int32)
+    getattr(lib(), "oracle_trans_xtoy" + _suffix(dt))(
+        C.byref(g), n, elm.ctypes.data_as(IP), idx.ctypes.data_as(IP), idy.ctypes.data_as(IP),
+        _ptr(lono, ct), _ptr(lato, ct), _ptr(levo, ct), _ptr(lon, ct), _ptr(lat, ct), _ptr(depth, ct),
+        _ptr(fsm, ct), _ptr(v3d, ct), _ptr(v2d, ct), _ptr(hx, ct))
+    return hx
+

@@ -23,7 +23,7 @@ FORM_PLAIN, FORM_BATCHED = 0, 1
 EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_da_main", "letkf_da_main_device", "letkf_core_batched",
            "letkf_bin_obs", "letkf_aoei", "letkf_da_adjoint_device",
-           "letkf_set_params", "letkf_get_stats", "letkf_last_error")
+           "letkf_set_params", "letkf_get_stats", "letkf_last_error", "letkf_trans_xtoy_device")
 
 
 class Params(C.Structure):
@@ -45,6 +45,15 @@ class Obs(C.Structure):
     _fields_ = [("nobs", C.c_int32), ("nobsgrd", C.c_void_p),
                 ("elm", C.c_void_p)] + [(n, C.c_void_p) for n in (
                     "lon", "lat", "lev", "err", "dep", "hdxf")]
+
+
+class XtoyGrid(C.Structure):
+    """letkf_xtoy_grid_t: the global grid and element constants of H(x)."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "nlon", "nlat", "nlev", "nv3d", "nv2d", "real_kind", "id_u_obs", "id_v_obs", "id_t_obs",
+        "id_s_obs", "id_z_obs", "iv3d_u", "iv3d_v", "iv3d_t", "iv3d_s", "iv2d_z")] + [
+        ("hnosea", C.c_double), ("undef", C.c_double)] + [
+        (n, C.c_void_p) for n in ("lon", "lat", "depth", "fsm")]
 
 
 class Stats(C.Structure):
@@ -91,6 +100,8 @@ def lib():
         L.letkf_da_adjoint_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp,
                                               vp, vp, vp, vp]
         L.letkf_set_params.argtypes = [vp, C.POINTER(Params)]
+        L.letkf_trans_xtoy_device.argtypes = [C.POINTER(XtoyGrid), C.c_int32, vp, vp, vp, vp, vp, vp,
+                                              C.c_int32, vp, vp, vp, vp, vp]
         L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.letkf_last_error.restype = C.c_char_p
         for n in EXPORTS:

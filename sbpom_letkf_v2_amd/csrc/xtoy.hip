@@ -1,0 +1,218 @@
+// xtoy.hip -- the observation operator H(x) of the reference,
+// Trans_XtoY (letkf/run/src/common.f90:391-500) with its bilinear_interpolation
+// (:246-269) and akima_spline (:276-385), batched over ensemble members: the
+// step that produces hdxf (common_io.f90:829-831, one call per member there).
+//
+// One thread per (obs, member), obs fastest, so that a wavefront reads the
+// neighbouring obs of one member (obs arrive sorted by grid cell).  Every
+// value follows the reference's operation order in r_size, with the double
+// literals of akima_spline promoted to FP64 as the Fortran promotes them;
+// contraction into FMA is off, so results are bit-identical to the CPU
+// restatement of the same routines.
+#include "letkf_handle.h"
+
+namespace letkf {
+
+namespace {
+
+template <typename T>
+struct XtoyArgs {
+  int nlon, nlat, nlev, nv3d, nv2d;
+  int id_u, id_v, id_t, id_s, id_z;
+  int iv_u, iv_v, iv_t, iv_s, iv_z;
+  T hnosea, undef;
+  const T *lon, *lat, *depth, *fsm;
+  int nobs, nmem;
+  const int32_t *elm, *idx, *idy;
+  const T *lono, *lato, *levo;
+  const T *v3d, *v2d;  // member m at v3d + m*nlon*nlat*nlev*nv3d, v2d + m*nlon*nlat*nv2d
+  T* hx;               // (nobs, nmem)
+  int32_t* qc;         // 0 where any member's H(x) is undef
+};
+
+template <typename T>
+__device__ __forceinline__ T bilinear(T x0, T x1, T y0, T y1, T x, T y, T f00, T f10, T f01, T f11, T undef) {
+#pragma clang fp contract(off)
+  if (x0 == x1 || y0 == y1) return undef;
+  return (y1 - y) / (y1 - y0) * ((x1 - x) / (x1 - x0) * f00 + (x - x0) / (x1 - x0) * f10) +
+         (y - y0) / (y1 - y0) * ((x1 - x) / (x1 - x0) * f01 + (x - x0) / (x1 - x0) * f11);
+}
+
+// akima_spline at one target on x(1:ndim), y(1:ndim) read with stride st
+// (a model column, levels 2..nlev).  A target equal to x(ndim) ends the
+// reference's bracket search past the column (it then reads beyond x and y
+// with dx = 0); y(ndim) is its value wherever those reads are finite and is
+// returned here.
+template <typename T>
+__device__ T akima(int ndim, const T* __restrict__ x, const T* __restrict__ y, long long st, T x5, T undef) {
+#pragma clang fp contract(off)
+#define AX(k) x[((long long)(k) - 1) * st]
+#define AY(k) y[((long long)(k) - 1) * st]
+#define DYDX(m) ((AY(i - 3 + (m)) - AY(i - 4 + (m))) / (AX(i - 3 + (m)) - AX(i - 4 + (m))))
+  if (x5 < AX(1) || AX(ndim) < x5) return undef;
+  int iflg = 0, i;
+  T y5 = undef;
+  for (i = 1; i <= ndim; ++i) {
+    const T xi = AX(i);
+    if (x5 == xi) {
+      y5 = AY(i);
+      iflg = 1;
+      continue;
+    }
+    if (x5 < xi) {
+      iflg = 1;
+      break;
+    }
+  }
+  if (i > ndim) return y5;
+  T dydx[6], ddydx[5], t[3];
+  if (i == 2) {
+    for (int m = 3; m <= 5; ++m) dydx[m] = DYDX(m);
+    dydx[2] = (T)(2.0 * (double)dydx[3] - (double)dydx[4]);
+    dydx[1] = (T)(2.0 * (double)dydx[2] - (double)dydx[3]);
+  } else if (i == 3) {
+    for (int m = 2; m <= 5; ++m) dydx[m] = DYDX(m);
+    dydx[1] = (T)(2.0 * (double)dydx[2] - (double)dydx[3]);
+  } else if (i == ndim) {
+    for (int m = 1; m <= 3; ++m) dydx[m] = DYDX(m);
+    dydx[4] = (T)(2.0 * (double)dydx[3] - (double)dydx[2]);
+    dydx[5] = (T)(2.0 * (double)dydx[4] - (double)dydx[3]);
+  } else if (i == ndim - 1) {
+    for (int m = 1; m <= 4; ++m) dydx[m] = DYDX(m);
+    dydx[5] = (T)(2.0 * (double)dydx[4] - (double)dydx[3]);
+  } else {
+    for (int m = 1; m <= 5; ++m) dydx[m] = DYDX(m);
+  }
+  for (int m = 1; m <= 4; ++m) {
+    const T dd = dydx[m + 1] - dydx[m];
+    ddydx[m] = dd < (T)0 ? -dd : dd;
+  }
+  for (int m = 1; m <= 2; ++m) {
+    const T wk = ddydx[m + 2] + ddydx[m];
+    t[m] = wk == (T)0 ? (T)0 : (ddydx[m + 2] * dydx[m + 1] + ddydx[m] * dydx[m + 2]) / wk;
+  }
+  const T dx21 = AX(i) - AX(i - 1), dx = x5 - AX(i - 1);
+  if (iflg == 0) return undef;
+  double v = (double)(T)(AY(i - 1) + dx * t[1]);
+  v = v + (double)(dx * dx) * (3.0 * (double)dydx[3] - 2.0 * (double)t[1] - (double)t[2]) / (double)dx21;
+  v = v + (double)(dx * dx * dx) * ((double)(t[1] + t[2]) - 2.0 * (double)dydx[3]) / (double)dx21 / (double)dx21;
+  return (T)v;
+#undef DYDX
+#undef AX
+#undef AY
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_qc_fill(int n, int32_t* qc) {
+  const int o = blockIdx.x * 256 + threadIdx.x;
+  if (o < n) qc[o] = 1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_trans_xtoy(XtoyArgs<T> a) {
+#pragma clang fp contract(off)
+  const long long tid = blockIdx.x * 256LL + threadIdx.x;
+  if (tid >= (long long)a.nobs * a.nmem) return;
+  const int o = (int)(tid % a.nobs), mem = (int)(tid / a.nobs);
+  const long long nij = (long long)a.nlon * a.nlat;
+  const T undef = a.undef;
+  T h = undef;
+  const int i = a.idx[o], j = a.idy[o];
+#define G2(p, ii, jj) (p)[((ii) - 1) + (long long)a.nlon * ((jj) - 1)]
+#define G3(p, ii, jj, kk) (p)[((ii) - 1) + (long long)a.nlon * ((jj) - 1) + nij * ((kk) - 1)]
+  do {
+    if (i < 1 || a.nlon - 1 < i || j < 1 || a.nlat - 1 < j) break;
+    if (G2(a.fsm, i, j) == a.hnosea || G2(a.fsm, i + 1, j) == a.hnosea || G2(a.fsm, i, j + 1) == a.hnosea ||
+        G2(a.fsm, i + 1, j + 1) == a.hnosea)
+      break;
+    const T x0 = G2(a.lon, i, j), x1 = G2(a.lon, i + 1, j), y0 = G2(a.lat, i, j), y1 = G2(a.lat, i, j + 1);
+    const int el = a.elm[o];
+    if (el == a.id_z) {
+      const T* f = a.v2d + nij * ((long long)mem * a.nv2d + (a.iv_z - 1));
+      h = bilinear<T>(x0, x1, y0, y1, a.lono[o], a.lato[o], G2(f, i, j), G2(f, i + 1, j), G2(f, i, j + 1),
+                      G2(f, i + 1, j + 1), undef);
+      break;
+    }
+    const int iv = el == a.id_u ? a.iv_u : el == a.id_v ? a.iv_v : el == a.id_t ? a.iv_t : el == a.id_s ? a.iv_s : 0;
+    if (iv == 0) break;  // an element Trans_XtoY does not know
+    const T lv = a.levo[o];
+    if ((T)0 < lv) break;  // above the sea surface
+    if (lv < G3(a.depth, i, j, 2) || lv < G3(a.depth, i + 1, j, 2) || lv < G3(a.depth, i, j + 1, 2) ||
+        lv < G3(a.depth, i + 1, j + 1, 2))
+      break;  // below the sea bottom
+    const T* f = a.v3d + nij * a.nlev * ((long long)mem * a.nv3d + (iv - 1));
+    const int ci[4] = {i, i + 1, i, i + 1}, cj[4] = {j, j, j + 1, j + 1};
+    T c[4];
+    if (G3(a.depth, i, j, a.nlev) < lv || G3(a.depth, i + 1, j, a.nlev) < lv ||
+        G3(a.depth, i, j + 1, a.nlev) < lv || G3(a.depth, i + 1, j + 1, a.nlev) < lv) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) c[q] = G3(f, ci[q], cj[q], a.nlev);
+    } else {
+      for (int q = 0; q < 4; ++q)
+        c[q] = akima<T>(a.nlev - 1, &G3(a.depth, ci[q], cj[q], 2), &G3(f, ci[q], cj[q], 2), nij, lv, undef);
+    }
+    h = bilinear<T>(x0, x1, y0, y1, a.lono[o], a.lato[o], c[0], c[1], c[2], c[3], undef);
+  } while (false);
+#undef G2
+#undef G3
+  a.hx[o + (long long)a.nobs * mem] = h;
+  if (a.qc && h == undef) a.qc[o] = 0;
+}
+
+template <typename T>
+int run_xtoy(const letkf_xtoy_grid_t* g, int nobs, const int32_t* elm, const int32_t* idx, const int32_t* idy,
+             const void* lono, const void* lato, const void* levo, int nmem, const void* v3d, const void* v2d,
+             void* hx, int32_t* qc, hipStream_t s) {
+  XtoyArgs<T> a;
+  a.nlon = g->nlon; a.nlat = g->nlat; a.nlev = g->nlev; a.nv3d = g->nv3d; a.nv2d = g->nv2d;
+  a.id_u = g->id_u_obs; a.id_v = g->id_v_obs; a.id_t = g->id_t_obs; a.id_s = g->id_s_obs; a.id_z = g->id_z_obs;
+  a.iv_u = g->iv3d_u; a.iv_v = g->iv3d_v; a.iv_t = g->iv3d_t; a.iv_s = g->iv3d_s; a.iv_z = g->iv2d_z;
+  a.hnosea = (T)g->hnosea;
+  a.undef = (T)g->undef;
+  a.lon = (const T*)g->lon; a.lat = (const T*)g->lat; a.depth = (const T*)g->depth; a.fsm = (const T*)g->fsm;
+  a.nobs = nobs; a.nmem = nmem;
+  a.elm = elm; a.idx = idx; a.idy = idy;
+  a.lono = (const T*)lono; a.lato = (const T*)lato; a.levo = (const T*)levo;
+  a.v3d = (const T*)v3d; a.v2d = (const T*)v2d;
+  a.hx = (T*)hx; a.qc = qc;
+  if (qc) {
+    hipLaunchKernelGGL(k_qc_fill<T>, dim3((unsigned)((nobs + 255) / 256)), dim3(256), 0, s, nobs, qc);
+    if (hipGetLastError() != hipSuccess) return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: launch failed");
+  }
+  const long long n = (long long)nobs * nmem;
+  hipLaunchKernelGGL(k_trans_xtoy<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: %s", hipGetErrorString(e));
+  return LETKF_OK;
+}
+
+}  // namespace
+
+}  // namespace letkf
+
+extern "C" int letkf_trans_xtoy_device(const letkf_xtoy_grid_t* g, int32_t nobs, const int32_t* elm,
+                                       const int32_t* idx, const int32_t* idy, const void* lono, const void* lato,
+                                       const void* levo, int32_t nmem, const void* v3d, const void* v2d, void* hx,
+                                       int32_t* qc, void* stream) {
+  using namespace letkf;
+  if (!g) return set_err(LETKF_ERR_ARG, "letkf_trans_xtoy_device: grid is NULL");
+  if (g->real_kind != 4 && g->real_kind != 8)
+    return set_err(LETKF_ERR_ARG, "real_kind must be 4 or 8 (got %d)", g->real_kind);
+  if (g->nlon < 2 || g->nlat < 2 || g->nlev < 6 || g->nv3d < 1 || g->nv2d < 1)
+    return set_err(LETKF_ERR_ARG, "grid (%d,%d,%d) nv3d %d nv2d %d: need nlon, nlat >= 2, nlev >= 6 (the Akima "
+                                  "stencil), nv3d, nv2d >= 1", g->nlon, g->nlat, g->nlev, g->nv3d, g->nv2d);
+  const int iv[4] = {g->iv3d_u, g->iv3d_v, g->iv3d_t, g->iv3d_s};
+  for (int q = 0; q < 4; ++q)
+    if (iv[q] < 1 || iv[q] > g->nv3d) return set_err(LETKF_ERR_ARG, "iv3d slot %d out of 1..%d", iv[q], g->nv3d);
+  if (g->iv2d_z < 1 || g->iv2d_z > g->nv2d)
+    return set_err(LETKF_ERR_ARG, "iv2d_z %d out of 1..%d", g->iv2d_z, g->nv2d);
+  if (nobs < 0 || nmem < 1) return set_err(LETKF_ERR_ARG, "nobs %d, nmem %d", nobs, nmem);
+  if (nobs == 0) return LETKF_OK;
+  if (!elm || !idx || !idy || !lono || !lato || !levo || !v3d || !v2d || !hx || !g->lon || !g->lat || !g->depth ||
+      !g->fsm)
+    return set_err(LETKF_ERR_ARG, "letkf_trans_xtoy_device: NULL array");
+  hipStream_t s = (hipStream_t)stream;
+  return g->real_kind == 8 ? run_xtoy<double>(g, nobs, elm, idx, idy, lono, lato, levo, nmem, v3d, v2d, hx, qc, s)
+                           : run_xtoy<float>(g, nobs, elm, idx, idy, lono, lato, levo, nmem, v3d, v2d, hx, qc, s);
+}

@@ -175,6 +175,40 @@ def _ptr(a):
     return C.c_void_p(a.data_ptr()) if a is not None and a.numel() else None
 
 
+def trans_xtoy_device(lon, lat, depth, fsm, elm, idx, idy, lono, lato, levo, v3d, v2d, qc=True,
+                      id_obs=(2819, 2820, 3073, 3332, 2567), iv3d=(1, 2, 3, 4), iv2d_z=1, hnosea=0.0,
+                      undef=999.0):
+    """H(x) of every member (letkf_trans_xtoy_device; Trans_XtoY,
+    letkf/run/src/common.f90:391-500, batched over members).
+
+    Device tensors in the reference's Fortran layouts, held as C-contiguous
+    torch tensors with the axes reversed: lon, lat, fsm (nlat, nlon); depth
+    (nlev, nlat, nlon); v3d (nmem, nv3d, nlev, nlat, nlon); v2d (nmem, nv2d,
+    nlat, nlon); obs vectors (nobs).  Returns hx (nmem, nobs), i.e. hx(nobs,
+    nmem) of the reference, and qc (nobs, int32) or None.  id_obs are the
+    element ids of u, v, t, s, SSH; iv3d the 1-based slots of u, v, t, s."""
+    import torch
+    dt = v3d.dtype
+    nmem, nv3d, nlev, nlat, nlon = v3d.shape
+    nv2d = v2d.shape[1]
+    for a in (lon, lat, depth, fsm, lono, lato, levo, v2d):
+        if a.dtype != dt or not a.is_contiguous() or not a.is_cuda:
+            raise ValueError("real arrays must be contiguous device tensors of one dtype")
+    for a in (elm, idx, idy):
+        if a.dtype != torch.int32 or not a.is_contiguous() or not a.is_cuda:
+            raise ValueError("elm/idx/idy must be contiguous int32 device tensors")
+    nobs = int(elm.numel())
+    g = _lib.XtoyGrid(nlon, nlat, nlev, nv3d, nv2d, 8 if dt == torch.float64 else 4, *id_obs, *iv3d, iv2d_z,
+                      float(hnosea), float(undef), _ptr(lon), _ptr(lat), _ptr(depth), _ptr(fsm))
+    hx = torch.empty((nmem, nobs), dtype=dt, device=v3d.device)
+    q = torch.empty(nobs, dtype=torch.int32, device=v3d.device) if qc else None
+    s = C.c_void_p(torch.cuda.current_stream(v3d.device).cuda_stream)
+    _lib.check(_lib.lib().letkf_trans_xtoy_device(C.byref(g), nobs, _ptr(elm), _ptr(idx), _ptr(idy),
+                                                   _ptr(lono), _ptr(lato), _ptr(levo), nmem, _ptr(v3d),
+                                                   _ptr(v2d), _ptr(hx), _ptr(q), s))
+    return hx, q
+
+
 def da_main(pr, precision="fp64", form="plain"):
     """Analysis of one problem; returns (anal3d, anal2d) in the same kind
     of container as the inputs."""
