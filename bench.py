@@ -50,10 +50,97 @@ def parse():
                     help="differentiable workload (default c3): time forward + hand-written adjoint and "
                          "check it by central finite differences")
     ap.add_argument("--fd-coords", type=int, default=200)
+    ap.add_argument("--xtoy", action="store_true",
+                    help="time the observation operator H(x) of every member (default workload c1) on the "
+                         "analysis arrays, with an oracle check and CPU baseline on a few members")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise kernel that reads and writes 1 GiB (three launches), the "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     return ap.parse_args()
+
+
+def run_xtoy(args):
+    """H(x) (Trans_XtoY, letkf/run/src/common.f90:391-500) of all members of a
+    workload's ensemble, read in place from the analysis arrays.  One step =
+    one call over every (obs, member).  Roofline: HBM, algorithmic bytes
+    = per (obs, member) the 4 x 6 column values an Akima obs reads (4 for
+    SSH and surface-level obs) plus the hx store; depth columns are shared
+    by all members and not counted."""
+    import numpy as np
+    import torch
+    import pyoracle as po
+    from sbpom_letkf_v2_amd import engine, synth
+    wl = args.workload if args.workload != "c2" else "c1"
+    log(f"generating workload {wl} on the GPU")
+    pr = synth.config(wl, device="cuda", real_kind=8)
+    meta = pr.meta
+    nlon, nlat, nlev, K = meta["nlon"], meta["nlat"], meta["nlev"], meta["nmem"]
+    nv3, nv2 = meta["nv3d"], meta["nv2d"]
+    lon = pr.lon1.reshape(nlat, nlon).contiguous()
+    lat = pr.lat1.reshape(nlat, nlon).contiguous()
+    depth = pr.depth1.reshape(nlev, nlat, nlon).contiguous()
+    fsm = torch.ones_like(lon)  # all sea: every obs does its full interpolation
+    dlon = float(lon[0, 1] - lon[0, 0])
+    dlat = float(lat[1, 0] - lat[0, 0])
+    idx = (torch.floor((pr.obslon - lon[0, 0]) / dlon) + 1).to(torch.int32).contiguous()
+    idy = (torch.floor((pr.obslat - lat[0, 0]) / dlat) + 1).to(torch.int32).contiguous()
+    elm, lono, lato, levo = pr.obselm, pr.obslon, pr.obslat, pr.obslev
+    v2 = pr.fcst2d if nv2 > 0 else torch.zeros((1, K, nlat * nlon), dtype=torch.float64, device="cuda")
+    nobs = int(elm.numel())
+    call = lambda: engine.trans_xtoy_device(lon, lat, depth, fsm, elm, idx, idy, lono, lato, levo, pr.fcst3d, v2,
+                                            analysis_layout=True)
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        hx, qc = call()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    units = nobs * K
+    undef = 999.0
+    ssh = (elm == synth.ID_Z).sum().item()
+    per_obs = torch.where(elm == synth.ID_Z, 5, 25).to(torch.float64)
+    bytes_step = float((per_obs.sum() * 8 * K).item())
+    gbs = bytes_step / (ms * 1e-3) / 1e9
+    defined = (hx != undef).float().mean().item()
+    log(f"  {ms:.3f} ms per call, {units / (ms * 1e-3):.3e} obs x members/s, {gbs:.0f} GB/s algorithmic, "
+        f"{defined:.3f} defined")
+    # oracle on a few members: bit-exact check at full size and the CPU baseline
+    nchk = 4
+    cpu = None
+    if not args.no_cpu:
+        host = lambda t: t.cpu().numpy()
+        g = po.OracleXtoY(nlon, nlat, nlev, nv3, max(nv2, 1), synth.ID_U, synth.ID_V, synth.ID_T, synth.ID_S,
+                          synth.ID_Z, 1, 2, 3, 4, 1, 0.0, undef)
+        cargs = [host(a) for a in (elm, idx, idy, lono, lato, levo)]
+        lonh, lath, deph, fsmh = host(lon), host(lat), host(depth), host(fsm)
+        mism, t_cpu = 0, 0.0
+        for m in range(nchk):
+            mm = (m * (K - 1)) // max(nchk - 1, 1)
+            v3m = np.ascontiguousarray(host(pr.fcst3d[:, mm]).reshape(nv3, nlev, nlat, nlon))
+            v2m = np.ascontiguousarray(host(v2[:, mm]).reshape(-1, nlat, nlon))
+            t0 = time.perf_counter()
+            want = po.trans_xtoy(g, *cargs, lonh.T, lath.T, deph.T, fsmh.T, v3m.T, v2m.T)
+            t_cpu += time.perf_counter() - t0
+            mism += int((host(hx[mm]) != want).sum())
+        cpu = {"value": nchk * nobs / t_cpu, "unit": "obs x members/s", "cores": 1, "kind": "port",
+               "sample": f"{nchk} members x {nobs} obs, FP64 C oracle (one thread), {t_cpu:.1f} s"}
+        log(f"  oracle on {nchk} members: {mism} mismatching values; cpu {cpu['value']:.3e} obs x members/s")
+        if mism:
+            raise SystemExit("H(x) differs from the oracle")
+    out = {"metric": "H(x) observations x members per second", "value": units / (ms * 1e-3),
+           "unit": "obs x members/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"{wl}: {nlon}x{nlat}x{nlev} K={K} nobs={nobs} ({ssh} SSH), H(x) of every member "
+                                  "from the analysis arrays"},
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": gbs / PEAK_HBM_GBS, "traffic": None},
+           "cpu_baseline": cpu, "defined_fraction": defined}
+    print(json.dumps(out))
 
 
 def calibrate_copy():
@@ -291,6 +378,9 @@ def main():
         return
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.xtoy:
+        run_xtoy(args)
         return
     import numpy as np
     import torch

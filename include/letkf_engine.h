@@ -203,8 +203,14 @@ typedef struct letkf_xtoy_grid {
 /* H(x) of every member on DEVICE arrays: Trans_XtoY (common.f90:391-500)
  * called by set_obs once per member (common_io.f90:829-831), batched.
  * elm/idx/idy/lono/lato/levo describe nobs obs (idx, idy 1-based cells);
- * v3d (nlon,nlat,nlev,nv3d,nmem) and v2d (nlon,nlat,nv2d,nmem) hold the
- * members; hx (nobs,nmem) receives H(x_m), undef where the reference gives
+ * v3d and v2d hold the members: element (i,j,k,v,m) of v3d at
+ * (i-1) + nlon (j-1) + nlon nlat (k-1) + s[0] (v-1) + s[1] (m-1), of v2d
+ * at (i-1) + nlon (j-1) + s[2] (v-1) + s[3] (m-1), with s = strides
+ * (elements); strides NULL or zero entries give one packed
+ * (nlon,nlat,nlev,nv3d) / (nlon,nlat,nv2d) block per member, and the
+ * analysis layout fcst3d(nij1,nlev,nmem,nv3d) with nij1 = nlon nlat is
+ * s = {nij1 nlev nmem, nij1 nlev, nij1 nmem, nij1}.  hx (nobs,nmem)
+ * receives H(x_m), undef where the reference gives
  * undef; qc (nobs, may be NULL) receives 0 where any member is undef (the
  * tmpqc rule of common_io.f90:833-839), 1 elsewhere.  Requires nlev >= 6
  * (the Akima stencil).  Runs on `stream` (NULL: the default stream) and
@@ -214,8 +220,9 @@ int letkf_trans_xtoy_device(const letkf_xtoy_grid_t* g, int32_t nobs,
                             const int32_t* elm, const int32_t* idx,
                             const int32_t* idy, const void* lono,
                             const void* lato, const void* levo, int32_t nmem,
-                            const void* v3d, const void* v2d, void* hx,
-                            int32_t* qc, void* stream);
+                            const void* v3d, const void* v2d,
+                            const int64_t* strides, void* hx, int32_t* qc,
+                            void* stream);
 
 /* Adjoint of letkf_da_main_device (no counterpart in the reference; for
  * gradient-based uses of the analysis: observation impact, tuning of the

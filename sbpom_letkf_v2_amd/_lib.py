@@ -101,7 +101,7 @@ def lib():
                                               vp, vp, vp, vp]
         L.letkf_set_params.argtypes = [vp, C.POINTER(Params)]
         L.letkf_trans_xtoy_device.argtypes = [C.POINTER(XtoyGrid), C.c_int32, vp, vp, vp, vp, vp, vp,
-                                              C.c_int32, vp, vp, vp, vp, vp]
+                                              C.c_int32, vp, vp, C.POINTER(C.c_int64), vp, vp, vp]
         L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.letkf_last_error.restype = C.c_char_p
         for n in EXPORTS:

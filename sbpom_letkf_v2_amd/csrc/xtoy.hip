@@ -25,7 +25,8 @@ struct XtoyArgs {
   int nobs, nmem;
   const int32_t *elm, *idx, *idy;
   const T *lono, *lato, *levo;
-  const T *v3d, *v2d;  // member m at v3d + m*nlon*nlat*nlev*nv3d, v2d + m*nlon*nlat*nv2d
+  const T *v3d, *v2d;  // element (i,j,k,v,m) at i + nlon j + nlon nlat k + sv3 v + sm3 m (v2d: no k)
+  long long sv3, sm3, sv2, sm2;
   T* hx;               // (nobs, nmem)
   int32_t* qc;         // 0 where any member's H(x) is undef
 };
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(256) k_trans_xtoy(XtoyArgs<T> a) {
     const T x0 = G2(a.lon, i, j), x1 = G2(a.lon, i + 1, j), y0 = G2(a.lat, i, j), y1 = G2(a.lat, i, j + 1);
     const int el = a.elm[o];
     if (el == a.id_z) {
-      const T* f = a.v2d + nij * ((long long)mem * a.nv2d + (a.iv_z - 1));
+      const T* f = a.v2d + a.sm2 * mem + a.sv2 * (a.iv_z - 1);
       h = bilinear<T>(x0, x1, y0, y1, a.lono[o], a.lato[o], G2(f, i, j), G2(f, i + 1, j), G2(f, i, j + 1),
                       G2(f, i + 1, j + 1), undef);
       break;
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(256) k_trans_xtoy(XtoyArgs<T> a) {
     if (lv < G3(a.depth, i, j, 2) || lv < G3(a.depth, i + 1, j, 2) || lv < G3(a.depth, i, j + 1, 2) ||
         lv < G3(a.depth, i + 1, j + 1, 2))
       break;  // below the sea bottom
-    const T* f = a.v3d + nij * a.nlev * ((long long)mem * a.nv3d + (iv - 1));
+    const T* f = a.v3d + a.sm3 * mem + a.sv3 * (iv - 1);
     const int ci[4] = {i, i + 1, i, i + 1}, cj[4] = {j, j, j + 1, j + 1};
     T c[4];
     if (G3(a.depth, i, j, a.nlev) < lv || G3(a.depth, i + 1, j, a.nlev) < lv ||
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(256) k_trans_xtoy(XtoyArgs<T> a) {
 template <typename T>
 int run_xtoy(const letkf_xtoy_grid_t* g, int nobs, const int32_t* elm, const int32_t* idx, const int32_t* idy,
              const void* lono, const void* lato, const void* levo, int nmem, const void* v3d, const void* v2d,
-             void* hx, int32_t* qc, hipStream_t s) {
+             const int64_t* strides, void* hx, int32_t* qc, hipStream_t s) {
   XtoyArgs<T> a;
   a.nlon = g->nlon; a.nlat = g->nlat; a.nlev = g->nlev; a.nv3d = g->nv3d; a.nv2d = g->nv2d;
   a.id_u = g->id_u_obs; a.id_v = g->id_v_obs; a.id_t = g->id_t_obs; a.id_s = g->id_s_obs; a.id_z = g->id_z_obs;
@@ -174,6 +175,12 @@ int run_xtoy(const letkf_xtoy_grid_t* g, int nobs, const int32_t* elm, const int
   a.elm = elm; a.idx = idx; a.idy = idy;
   a.lono = (const T*)lono; a.lato = (const T*)lato; a.levo = (const T*)levo;
   a.v3d = (const T*)v3d; a.v2d = (const T*)v2d;
+  const long long nij = (long long)g->nlon * g->nlat;
+  // default: one packed (nlon,nlat,nlev,nv3d) / (nlon,nlat,nv2d) block per member
+  a.sv3 = strides && strides[0] ? strides[0] : nij * g->nlev;
+  a.sm3 = strides && strides[1] ? strides[1] : nij * g->nlev * g->nv3d;
+  a.sv2 = strides && strides[2] ? strides[2] : nij;
+  a.sm2 = strides && strides[3] ? strides[3] : nij * g->nv2d;
   a.hx = (T*)hx; a.qc = qc;
   if (qc) {
     hipLaunchKernelGGL(k_qc_fill<T>, dim3((unsigned)((nobs + 255) / 256)), dim3(256), 0, s, nobs, qc);
@@ -193,8 +200,8 @@ int run_xtoy(const letkf_xtoy_grid_t* g, int nobs, const int32_t* elm, const int
 
 extern "C" int letkf_trans_xtoy_device(const letkf_xtoy_grid_t* g, int32_t nobs, const int32_t* elm,
                                        const int32_t* idx, const int32_t* idy, const void* lono, const void* lato,
-                                       const void* levo, int32_t nmem, const void* v3d, const void* v2d, void* hx,
-                                       int32_t* qc, void* stream) {
+                                       const void* levo, int32_t nmem, const void* v3d, const void* v2d,
+                                       const int64_t* strides, void* hx, int32_t* qc, void* stream) {
   using namespace letkf;
   if (!g) return set_err(LETKF_ERR_ARG, "letkf_trans_xtoy_device: grid is NULL");
   if (g->real_kind != 4 && g->real_kind != 8)
@@ -213,6 +220,10 @@ extern "C" int letkf_trans_xtoy_device(const letkf_xtoy_grid_t* g, int32_t nobs,
       !g->fsm)
     return set_err(LETKF_ERR_ARG, "letkf_trans_xtoy_device: NULL array");
   hipStream_t s = (hipStream_t)stream;
-  return g->real_kind == 8 ? run_xtoy<double>(g, nobs, elm, idx, idy, lono, lato, levo, nmem, v3d, v2d, hx, qc, s)
-                           : run_xtoy<float>(g, nobs, elm, idx, idy, lono, lato, levo, nmem, v3d, v2d, hx, qc, s);
+  if (strides)
+    for (int q = 0; q < 4; ++q)
+      if (strides[q] < 0) return set_err(LETKF_ERR_ARG, "letkf_trans_xtoy_device: negative stride");
+  return g->real_kind == 8
+             ? run_xtoy<double>(g, nobs, elm, idx, idy, lono, lato, levo, nmem, v3d, v2d, strides, hx, qc, s)
+             : run_xtoy<float>(g, nobs, elm, idx, idy, lono, lato, levo, nmem, v3d, v2d, strides, hx, qc, s);
 }

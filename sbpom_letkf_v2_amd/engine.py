@@ -177,20 +177,32 @@ def _ptr(a):
 
 def trans_xtoy_device(lon, lat, depth, fsm, elm, idx, idy, lono, lato, levo, v3d, v2d, qc=True,
                       id_obs=(2819, 2820, 3073, 3332, 2567), iv3d=(1, 2, 3, 4), iv2d_z=1, hnosea=0.0,
-                      undef=999.0):
+                      undef=999.0, analysis_layout=False):
     """H(x) of every member (letkf_trans_xtoy_device; Trans_XtoY,
     letkf/run/src/common.f90:391-500, batched over members).
 
     Device tensors in the reference's Fortran layouts, held as C-contiguous
     torch tensors with the axes reversed: lon, lat, fsm (nlat, nlon); depth
     (nlev, nlat, nlon); v3d (nmem, nv3d, nlev, nlat, nlon); v2d (nmem, nv2d,
-    nlat, nlon); obs vectors (nobs).  Returns hx (nmem, nobs), i.e. hx(nobs,
-    nmem) of the reference, and qc (nobs, int32) or None.  id_obs are the
-    element ids of u, v, t, s, SSH; iv3d the 1-based slots of u, v, t, s."""
+    nlat, nlon); obs vectors (nobs).  With analysis_layout the members are
+    the analysis arrays instead: v3d (nv3d, nmem, nlev, nlat*nlon) =
+    fcst3d(nij1,nlev,nmem,nv3d), v2d (nv2d, nmem, nlat*nlon) =
+    fcst2d(nij1,nmem,nv2d).  Returns hx (nmem, nobs), i.e. hx(nobs, nmem) of
+    the reference, and qc (nobs, int32) or None.  id_obs are the element ids
+    of u, v, t, s, SSH; iv3d the 1-based slots of u, v, t, s."""
     import torch
     dt = v3d.dtype
-    nmem, nv3d, nlev, nlat, nlon = v3d.shape
-    nv2d = v2d.shape[1]
+    strides = None
+    if analysis_layout:
+        nv3d, nmem, nlev, nij = v3d.shape
+        nv2d = v2d.shape[0]
+        nlat, nlon = lon.shape
+        if nij != nlat * nlon or v2d.shape[1] != nmem:
+            raise ValueError("analysis layout: v3d (nv3d, nmem, nlev, nlat*nlon), v2d (nv2d, nmem, nlat*nlon)")
+        strides = (C.c_int64 * 4)(nij * nlev * nmem, nij * nlev, nij * nmem, nij)
+    else:
+        nmem, nv3d, nlev, nlat, nlon = v3d.shape
+        nv2d = v2d.shape[1]
     for a in (lon, lat, depth, fsm, lono, lato, levo, v2d):
         if a.dtype != dt or not a.is_contiguous() or not a.is_cuda:
             raise ValueError("real arrays must be contiguous device tensors of one dtype")
@@ -205,7 +217,7 @@ def trans_xtoy_device(lon, lat, depth, fsm, elm, idx, idy, lono, lato, levo, v3d
     s = C.c_void_p(torch.cuda.current_stream(v3d.device).cuda_stream)
     _lib.check(_lib.lib().letkf_trans_xtoy_device(C.byref(g), nobs, _ptr(elm), _ptr(idx), _ptr(idy),
                                                    _ptr(lono), _ptr(lato), _ptr(levo), nmem, _ptr(v3d),
-                                                   _ptr(v2d), _ptr(hx), _ptr(q), s))
+                                                   _ptr(v2d), strides, _ptr(hx), _ptr(q), s))
     return hx, q
 
 

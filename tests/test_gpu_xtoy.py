@@ -56,3 +56,22 @@ def test_trans_xtoy_rejects_short_columns():
         engine.trans_xtoy_device(cu(lon), cu(lat), cu(depth), cu(fsm), one(xc.ID_T, torch.int32),
                                  one(2, torch.int32), one(2, torch.int32), one(120.3, torch.float64),
                                  one(10.3, torch.float64), one(-3.0, torch.float64), cu(v3d), cu(v2d))
+
+
+def test_trans_xtoy_analysis_layout_matches_packed():
+    """The same members read from the analysis arrays fcst3d(nij1,nlev,nmem,
+    nv3d) / fcst2d(nij1,nmem,nv2d) through explicit strides."""
+    _say("case xtoy analysis layout")
+    rng = np.random.default_rng(9)
+    nlon, nlat, nlev, nv3d, nv2d, nmem = 32, 24, 12, 5, 2, 6
+    lon, lat, fsm, depth = xc.make_grid(rng, nlon, nlat, nlev, ((1, nlon), (1, nlat)))
+    v3d, v2d = xc.make_fields(rng, nmem, nv3d, nv2d, nlev, nlat, nlon)
+    obs = xc.make_obs(rng, 2000, lon, lat, depth, ((1, nlon), (1, nlat)), nlon, nlat)
+    cu = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    args = [cu(lon), cu(lat), cu(depth), cu(fsm)] + [cu(a) for a in obs]
+    hx1, q1 = engine.trans_xtoy_device(*args, cu(v3d), cu(v2d))
+    a3 = np.ascontiguousarray(v3d.transpose(1, 0, 2, 3, 4).reshape(nv3d, nmem, nlev, nlat * nlon))
+    a2 = np.ascontiguousarray(v2d.transpose(1, 0, 2, 3).reshape(nv2d, nmem, nlat * nlon))
+    hx2, q2 = engine.trans_xtoy_device(*args, cu(a3), cu(a2), analysis_layout=True)
+    torch.cuda.synchronize()
+    assert torch.equal(hx1, hx2) and torch.equal(q1, q2)
