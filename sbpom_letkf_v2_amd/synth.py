@@ -327,6 +327,25 @@ def make_problem(nlon=64, nlat=64, nlev=16, nmem=20, nv3d=4, nv2d=1, nobs=2000,
     return prob
 
 
+# Workload c4: eight independent analysis cycles, grids 128^2 .. 512^2 x 40,
+# nmem round-robin over (32, 64, 100, 128), 0.5 .. 4 random obs per column.
+C4_GRIDS = (128, 192, 256, 256, 320, 384, 448, 512)
+C4_NMEM = (32, 64, 100, 128)
+C4_DENSITY = (0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0)
+
+
+def c4_cycles(scale=1.0, nlev=40):
+    """make_problem arguments of the eight c4 cycles; `scale` shrinks the
+    horizontal grid (the parity test runs them at a few % of full size)."""
+    out = []
+    for c, (n, d) in enumerate(zip(C4_GRIDS, C4_DENSITY)):
+        m = max(8, int(round(n * scale)))
+        out.append(dict(nlon=m, nlat=m, nlev=nlev, nmem=C4_NMEM[c % 4], nv3d=4, nv2d=1,
+                        nobs=max(1, int(d * m * m)), obs_layout="random", alpha_rtpp=0.9,
+                        seed=20261207 + c))
+    return out
+
+
 def config(name: str, device="cpu", real_kind=8, **kw) -> Problem:
     """Named workloads: c0 (64x64x16, K=20, 2000 random obs, RTPP 0.7),
     c1 (512x512x40, K=100, 6 vars incl. tracer, 1.5e5 swath + 5e4 profile

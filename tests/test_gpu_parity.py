@@ -251,3 +251,34 @@ def test_mixed_precision_gram_close_to_fp64(precision, form, K, nobs):
     e3 = per_var_err(a3, a3o, axis_var=3)
     _say(f"  err3={e3}")
     assert max(e3) <= 1e-4
+
+
+def test_c4_cycles_mixed_k():
+    """Workload c4 at reduced size: eight independent cycles, nmem round-robin
+    over 32, 64, 100, 128, 0.5 .. 4 obs per column, run in sequence on fresh
+    handles.  Each cycle's mixed-precision analysis (FP32 and BF16 hi/lo
+    Gram alternating) stays within 1e-4 of the FP64 engine on the same
+    cycle; the FP64 engine meets the oracle (1e-10) on the nmem <= 64 cycles."""
+    cycles = synth.c4_cycles(scale=1.0 / 16.0, nlev=6)
+    for c, kw in enumerate(cycles):
+        prec = "fp32" if c % 2 == 0 else "bf16x3"
+        _say(f"case c4 cycle {c} nlon={kw['nlon']} nmem={kw['nmem']} nobs={kw['nobs']} {prec}")
+        pr = synth.make_problem(device="cuda", real_kind=8, sigma_h_cells=1.5, sigma_v=1.0, **kw)
+        outs = {}
+        for name, pf in (("fp64", ("fp64", "batched")), ("mixed", (prec, "batched"))):
+            eng = engine.Engine(engine.make_params(pr, *pf))
+            a3, a2 = eng.da_main_device(pr)
+            torch.cuda.synchronize()
+            eng.close()
+            outs[name] = (synth._to_fortran(a3), synth._to_fortran(a2))
+        e3 = per_var_err(outs["mixed"][0], outs["fp64"][0], axis_var=3)
+        _say(f"  mixed vs FP64 err3={max(e3):.2e}")
+        assert max(e3) <= 1e-4
+        if kw["nmem"] <= 64:
+            opr = to_oracle(pr.to("cpu"))
+            rc, a3o, a2o, nl = po.da_main(opr, dtype=np.float64)
+            assert rc == 0
+            e = per_var_err(outs["fp64"][0], a3o, axis_var=3)
+            _say(f"  FP64 vs oracle err3={max(e):.2e}")
+            assert max(e) <= 1e-10
+
