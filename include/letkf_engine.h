@@ -224,6 +224,69 @@ int letkf_trans_xtoy_device(const letkf_xtoy_grid_t* g, int32_t nobs,
                             const int64_t* strides, void* hx, int32_t* qc,
                             void* stream);
 
+/* Settings of set_obs (common_setting.f90:56-63, 96, 101-118 and the
+ * gross_error parameter of common_obs.f90:35).  Bound classes of the
+ * gross-error check, in the reference's test order: 0 SSH, 1 SST (T at
+ * lev 0), 2 T, 3 SSS, 4 S, 5 SSU, 6 U, 7 SSV, 8 V; other elements are
+ * rejected when |dep| > gross_error * err. */
+typedef struct letkf_setobs_cfg {
+  int32_t id_u_obs, id_v_obs, id_t_obs, id_s_obs, id_z_obs;
+  int32_t lge_io; /* gross-error check on (applied only when aoei == 0) */
+  int32_t aoei;   /* adaptive obs-error inflation after the sort */
+  int32_t nslots; /* time slots; obs come ordered by slot */
+  int32_t nbslot; /* base slot (1-based) */
+  double sigma_obst, undef, gross_error;
+  double qc_min[9], qc_max[9];
+} letkf_setobs_cfg_t;
+
+/* Fills the reference's settings: LGE_IO off, AOEI on, one slot. */
+void letkf_setobs_default(letkf_setobs_cfg_t* c);
+
+/* Observation tables written by letkf_set_obs_device: DEVICE buffers of
+ * capacity nobs (hdxf nobs*nmem, nobsgrd nlon*nlat), the sorted obs first.
+ * hdxf is (nobs_out,nmem) column-major, element (o,m) at o + nobs_out*m:
+ * with nobsgrd, elm, lon, lat, lev, err, dep it is the letkf_obs_t that
+ * letkf_da_main_device takes.  ins, idx, idy, dat, hxfmean, hxfsprd may be
+ * NULL (not written).  nobs_out, ndel_proj and ndel_gross are written on
+ * the host: obs kept, obs dropped by the projection (qc 0 from H(x) or a
+ * cell outside the grid: numdelqc1) and by the gross-error check
+ * (numdelqc2). */
+typedef struct letkf_obs_set {
+  int32_t* nobsgrd;
+  int32_t* elm;
+  int32_t* ins;
+  int32_t* idx;
+  int32_t* idy;
+  void* lon;
+  void* lat;
+  void* lev;
+  void* dat;
+  void* err;
+  void* dep;
+  void* hxfmean;
+  void* hxfsprd;
+  void* hdxf;
+  int32_t nobs_out, ndel_proj, ndel_gross;
+} letkf_obs_set_t;
+
+/* The body of set_obs after H(x) (common_obs.f90:139-402) on DEVICE
+ * arrays of nobs unsorted obs (elm, lon, lat, lev, dat, err, idx, idy;
+ * ins may be NULL), hx (nobs,nmem) column-major and qc from
+ * letkf_trans_xtoy_device: ensemble mean, spread, perturbations and
+ * innovation per obs, gross-error check, temporal localisation of err,
+ * removal of qc 0 obs, stable sort by (idy, idx) with nobsgrd, AOEI.
+ * nobslots (HOST, nslots counts, may be NULL when nslots == 1) splits the
+ * obs into time slots.  Real arrays have the handle's real_kind; nmem is
+ * the handle's.  Returns when done. */
+int letkf_set_obs_device(letkf_handle_t* h, const letkf_setobs_cfg_t* cfg,
+                         int32_t nlon, int32_t nlat, int32_t nobs,
+                         const int32_t* nobslots, const int32_t* elm,
+                         const int32_t* ins, const void* lon, const void* lat,
+                         const void* lev, const void* dat, const void* err,
+                         const int32_t* idx, const int32_t* idy,
+                         const void* hx, const int32_t* qc,
+                         letkf_obs_set_t* out);
+
 /* Adjoint of letkf_da_main_device (no counterpart in the reference; for
  * gradient-based uses of the analysis: observation impact, tuning of the
  * inflation and of the observation errors, torch.autograd).  DEVICE arrays;

@@ -59,6 +59,17 @@ typedef struct oracle_xtoy {
   double hnosea, undef;
 } oracle_xtoy_t;
 
+/* Settings of set_obs (common_setting.f90:56-63,96,101-118;
+ * common_obs.f90:35).  Gross-error bound classes in the reference's test
+ * order: 0 SSH, 1 SST, 2 T, 3 SSS, 4 S, 5 SSU, 6 U, 7 SSV, 8 V. */
+typedef struct oracle_setobs {
+  int32_t nmem, nlon, nlat, nslots, nbslot;
+  int32_t id_u_obs, id_v_obs, id_t_obs, id_s_obs, id_z_obs;
+  int32_t lge_io, aoei;
+  double sigma_obst, undef, gross_error;
+  double qc_min[9], qc_max[9];
+} oracle_setobs_t;
+
 typedef struct oracle_tables_f64 { ORACLE_TABLES(double) } oracle_tables_f64_t;
 typedef struct oracle_tables_f32 { ORACLE_TABLES(float) } oracle_tables_f32_t;
 
@@ -99,7 +110,18 @@ typedef struct oracle_tables_f32 { ORACLE_TABLES(float) } oracle_tables_f32_t;
                               const REAL* lato, const REAL* levo,              \
                               const REAL* lon, const REAL* lat,                \
                               const REAL* depth, const REAL* fsm,              \
-                              const REAL* v3d, const REAL* v2d, REAL* hx);
+                              const REAL* v3d, const REAL* v2d, REAL* hx);     \
+  int oracle_set_obs##SUF(const oracle_setobs_t* c, int nobs,                  \
+                          const int32_t* nobslots, const int32_t* elm,         \
+                          const int32_t* ins, const REAL* lon,                 \
+                          const REAL* lat, const REAL* lev, const REAL* dat,   \
+                          const REAL* err, const int32_t* idx,                 \
+                          const int32_t* idy, const REAL* hx,                  \
+                          const int32_t* qc, int32_t* o_elm, int32_t* o_ins,   \
+                          int32_t* o_idx, int32_t* o_idy, REAL* o_lon,         \
+                          REAL* o_lat, REAL* o_lev, REAL* o_dat, REAL* o_err,  \
+                          REAL* o_dep, REAL* o_mean, REAL* o_sprd,             \
+                          REAL* o_hdxf, int32_t* nobsgrd, int32_t* ndel);
 
 ORACLE_DECL(_f64, double)
 ORACLE_DECL(_f32, float)

@@ -64,6 +64,34 @@ _lib = None
 _ref = None
 
 
+class OracleSetObs(C.Structure):
+    """oracle_setobs_t: settings of set_obs (common_setting.f90:56-63,
+    96, 101-118; common_obs.f90:35)."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "nmem", "nlon", "nlat", "nslots", "nbslot", "id_u_obs", "id_v_obs", "id_t_obs",
+        "id_s_obs", "id_z_obs", "lge_io", "aoei")] + [
+        (n, C.c_double) for n in ("sigma_obst", "undef", "gross_error")] + [
+        ("qc_min", C.c_double * 9), ("qc_max", C.c_double * 9)]
+
+
+def setobs_config(nmem, nlon, nlat, **kw) -> OracleSetObs:
+    """The reference's settings (LGE_IO off, AOEI on, one slot), overridden
+    by keyword (qc_min / qc_max as 9-sequences)."""
+    c = OracleSetObs(nmem=nmem, nlon=nlon, nlat=nlat, nslots=1, nbslot=1, id_u_obs=2819,
+                     id_v_obs=2820, id_t_obs=3073, id_s_obs=3332, id_z_obs=2567, lge_io=0, aoei=1,
+                     sigma_obst=1.0, undef=999.0, gross_error=10.0)
+    lo = (-1, -5, -5, -2, -2, -2, -2, -2, -2)
+    for k in range(9):
+        c.qc_min[k], c.qc_max[k] = lo[k], -lo[k]
+    for k, v in kw.items():
+        if k in ("qc_min", "qc_max"):
+            for i, x in enumerate(v):
+                getattr(c, k)[i] = x
+        else:
+            setattr(c, k, v)
+    return c
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -109,6 +137,10 @@ def lib():
             f = getattr(_lib, "oracle_trans_xtoy" + suf)
             f.restype = None
             f.argtypes = [C.POINTER(OracleXtoY), C.c_int, IP, IP, IP, P, P, P, P, P, P, P, P, P, P]
+            f = getattr(_lib, "oracle_set_obs" + suf)
+            f.restype = C.c_int
+            f.argtypes = ([C.POINTER(OracleSetObs), C.c_int, IP, IP, IP] + [P] * 5 + [IP, IP, P, IP]
+                          + [IP] * 4 + [P] * 9 + [IP, IP])
     return _lib
 
 
@@ -393,3 +425,46 @@ def trans_xtoy(g: OracleXtoY, elm, idx, idy, lono, lato, levo, lon, lat, depth, 
         _ptr(fsm, ct), _ptr(v3d, ct), _ptr(v2d, ct), _ptr(hx, ct))
     return hx
 
+
+
+def set_obs(c: OracleSetObs, elm, lon, lat, lev, dat, err, idx, idy, hx, qc=None, ins=None,
+            nobslots=None, dtype=np.float64):
+    """set_obs after H(x) (common_obs.f90:139-402); hx is (nobs, nmem).
+    Returns a dict of the sorted tables (hdxf (nobs_out, nmem)), nobsgrd
+    (nlon, nlat) and ndel = (numdelqc1, numdelqc2).  A nobslots sequence
+    sets c.nslots to its length."""
+    dt = np.dtype(dtype)
+    ct = _ct(dt)
+    n = len(elm)
+    IP = C.POINTER(C.c_int32)
+    I = lambda a: np.ascontiguousarray(a, np.int32)
+    R = lambda a: np.ascontiguousarray(a, dt)
+    elm, idx, idy = I(elm), I(idx), I(idy)
+    lon, lat, lev, dat, err = (R(a) for a in (lon, lat, lev, dat, err))
+    hxf = np.asfortranarray(np.asarray(hx, dt).reshape(n, c.nmem))
+    qca = None if qc is None else I(qc)
+    insa = None if ins is None else I(ins)
+    sl = None if nobslots is None else I(nobslots)
+    if sl is not None:
+        c.nslots = len(sl)
+    out = {k: np.zeros(max(n, 1), np.int32) for k in ("elm", "ins", "idx", "idy")}
+    out.update({k: np.zeros(max(n, 1), dt) for k in ("lon", "lat", "lev", "dat", "err", "dep",
+                                                      "hxfmean", "hxfsprd")})
+    hd = np.zeros(max(n, 1) * c.nmem, dt)
+    grd = np.zeros(c.nlon * c.nlat, np.int32)
+    ndel = np.zeros(2, np.int32)
+    ip = lambda a: None if a is None else a.ctypes.data_as(IP)
+    nout = getattr(lib(), "oracle_set_obs" + _suffix(dt))(
+        C.byref(c), n, ip(sl), ip(elm), ip(insa), _ptr(lon, ct), _ptr(lat, ct), _ptr(lev, ct),
+        _ptr(dat, ct), _ptr(err, ct), ip(idx), ip(idy), _ptr(hxf, ct), ip(qca),
+        ip(out["elm"]), ip(out["ins"]), ip(out["idx"]), ip(out["idy"]),
+        *(_ptr(out[k], ct) for k in ("lon", "lat", "lev", "dat", "err", "dep", "hxfmean", "hxfsprd")),
+        _ptr(hd, ct), ip(grd), ip(ndel))
+    if nout < 0:
+        raise ValueError("nobslots does not sum to nobs")
+    res = {k: v[:nout].copy() for k, v in out.items()}
+    res["hdxf"] = hd[:nout * c.nmem].reshape(c.nmem, nout).T.copy()
+    res["nobsgrd"] = grd.reshape(c.nlat, c.nlon).T.copy()
+    res["nobs"] = nout
+    res["ndel"] = (int(ndel[0]), int(ndel[1]))
+    return res

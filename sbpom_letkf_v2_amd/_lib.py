@@ -23,7 +23,8 @@ FORM_PLAIN, FORM_BATCHED = 0, 1
 EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_da_main", "letkf_da_main_device", "letkf_core_batched",
            "letkf_bin_obs", "letkf_aoei", "letkf_da_adjoint_device",
-           "letkf_set_params", "letkf_get_stats", "letkf_last_error", "letkf_trans_xtoy_device")
+           "letkf_set_params", "letkf_get_stats", "letkf_last_error", "letkf_trans_xtoy_device",
+           "letkf_setobs_default", "letkf_set_obs_device")
 
 
 class Params(C.Structure):
@@ -54,6 +55,21 @@ class XtoyGrid(C.Structure):
         "id_s_obs", "id_z_obs", "iv3d_u", "iv3d_v", "iv3d_t", "iv3d_s", "iv2d_z")] + [
         ("hnosea", C.c_double), ("undef", C.c_double)] + [
         (n, C.c_void_p) for n in ("lon", "lat", "depth", "fsm")]
+
+
+class SetObsCfg(C.Structure):
+    """letkf_setobs_cfg_t: settings of set_obs."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "id_u_obs", "id_v_obs", "id_t_obs", "id_s_obs", "id_z_obs", "lge_io", "aoei", "nslots",
+        "nbslot")] + [(n, C.c_double) for n in ("sigma_obst", "undef", "gross_error")] + [
+        ("qc_min", C.c_double * 9), ("qc_max", C.c_double * 9)]
+
+
+class ObsSet(C.Structure):
+    """letkf_obs_set_t: device outputs of letkf_set_obs_device."""
+    _fields_ = [(n, C.c_void_p) for n in (
+        "nobsgrd", "elm", "ins", "idx", "idy", "lon", "lat", "lev", "dat", "err", "dep", "hxfmean",
+        "hxfsprd", "hdxf")] + [(n, C.c_int32) for n in ("nobs_out", "ndel_proj", "ndel_gross")]
 
 
 class Stats(C.Structure):
@@ -102,10 +118,13 @@ def lib():
         L.letkf_set_params.argtypes = [vp, C.POINTER(Params)]
         L.letkf_trans_xtoy_device.argtypes = [C.POINTER(XtoyGrid), C.c_int32, vp, vp, vp, vp, vp, vp,
                                               C.c_int32, vp, vp, C.POINTER(C.c_int64), vp, vp, vp]
+        L.letkf_setobs_default.argtypes = [C.POINTER(SetObsCfg)]
+        L.letkf_set_obs_device.argtypes = ([vp, C.POINTER(SetObsCfg), C.c_int32, C.c_int32, C.c_int32,
+                                            C.POINTER(C.c_int32)] + [vp] * 11 + [C.POINTER(ObsSet)])
         L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.letkf_last_error.restype = C.c_char_p
         for n in EXPORTS:
-            if n != "letkf_last_error" and hasattr(L, n):
+            if n not in ("letkf_last_error", "letkf_setobs_default") and hasattr(L, n):
                 getattr(L, n).restype = C.c_int
         _lib = L
     return _lib

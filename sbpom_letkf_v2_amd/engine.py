@@ -151,6 +151,66 @@ class Engine:
                                               ptr(pr.fcst2d), ptr(anal3d), ptr(anal2d), s))
         return anal3d, anal2d
 
+    def set_obs_device(self, nlon, nlat, elm, lon, lat, lev, dat, err, idx, idy, hx, qc=None, ins=None,
+                       nobslots=None, **cfg):
+        """letkf_set_obs_device: set_obs after H(x) (common_obs.f90:139-402)
+        on device tensors of the nobs unsorted obs; hx (nmem, nobs) and qc as
+        trans_xtoy_device returns them.  cfg overrides the reference's
+        settings (lge_io, aoei, nslots, nbslot, sigma_obst, undef,
+        gross_error, id_*_obs, qc_min / qc_max as 9-sequences); nobslots is
+        a host sequence of per-slot counts.  Returns a dict of device tensors
+        named as the obs tables da_main_device reads (obselm, obslon, obslat,
+        obslev, obserr, obsdep, obshdxf (nmem, nobs_out), nobsgrd (nlat,
+        nlon)) plus obsins, obsidx, obsidy, obsdat, obshxfmean, obshxfsprd,
+        and the counts nobs, ndel_proj, ndel_gross."""
+        import torch
+        dt = torch.float32 if self.params.real_kind == 4 else torch.float64
+        nmem = int(self.params.nmem)
+        nobs = int(elm.numel())
+        for a in (lon, lat, lev, dat, err, hx):
+            if a.dtype != dt or not a.is_contiguous() or not a.is_cuda:
+                raise ValueError(f"real arrays must be contiguous {dt} device tensors")
+        for a in (elm, idx, idy) + tuple(x for x in (qc, ins) if x is not None):
+            if a.dtype != torch.int32 or not a.is_contiguous() or not a.is_cuda:
+                raise ValueError("integer arrays must be contiguous int32 device tensors")
+        if hx.numel() != nobs * nmem:
+            raise ValueError(f"hx must hold nmem*nobs = {nmem}*{nobs} values")
+        c = _lib.SetObsCfg()
+        _lib.lib().letkf_setobs_default(C.byref(c))
+        for k, v in cfg.items():
+            if k in ("qc_min", "qc_max"):
+                for i, x in enumerate(v):
+                    getattr(c, k)[i] = float(x)
+            elif hasattr(c, k):
+                setattr(c, k, v)
+            else:
+                raise TypeError(f"unknown set_obs setting {k}")
+        slots = None
+        if nobslots is not None:
+            nobslots = [int(x) for x in nobslots]
+            c.nslots = len(nobslots)
+            slots = (C.c_int32 * len(nobslots))(*nobslots)
+        dev = elm.device
+        cap = max(nobs, 1)
+        I = lambda: torch.empty(cap, dtype=torch.int32, device=dev)
+        R = lambda: torch.empty(cap, dtype=dt, device=dev)
+        o = {"obselm": I(), "obsins": I(), "obsidx": I(), "obsidy": I(), "obslon": R(), "obslat": R(),
+             "obslev": R(), "obsdat": R(), "obserr": R(), "obsdep": R(), "obshxfmean": R(), "obshxfsprd": R()}
+        hd = torch.empty(cap * nmem, dtype=dt, device=dev)
+        grd = torch.empty((nlat, nlon), dtype=torch.int32, device=dev)
+        out = _lib.ObsSet(_ptr(grd), *(_ptr(o[k]) for k in ("obselm", "obsins", "obsidx", "obsidy")),
+                          *(_ptr(o[k]) for k in ("obslon", "obslat", "obslev", "obsdat", "obserr", "obsdep",
+                                                 "obshxfmean", "obshxfsprd")), _ptr(hd), 0, 0, 0)
+        check(_lib.lib().letkf_set_obs_device(self.h, C.byref(c), nlon, nlat, nobs, slots, _ptr(elm), _ptr(ins),
+                                              _ptr(lon), _ptr(lat), _ptr(lev), _ptr(dat), _ptr(err), _ptr(idx),
+                                              _ptr(idy), _ptr(hx), _ptr(qc), C.byref(out)))
+        n = int(out.nobs_out)
+        res = {k: v[:n] for k, v in o.items()}
+        res["obshdxf"] = hd[:n * nmem].view(nmem, n)
+        res["nobsgrd"] = grd
+        res.update(nobs=n, ndel_proj=int(out.ndel_proj), ndel_gross=int(out.ndel_gross))
+        return res
+
     def da_adjoint_device(self, pr, g3, g2, stream=None):
         """letkf_da_adjoint_device: (dL/d obsdep, dL/d log obserr^2,
         [dL/d alpha_rtpp, dL/d cov_infl_mul]) for upstream gradients g3, g2 of

@@ -19,7 +19,7 @@ HEADER = os.path.join(ROOT, "include", "letkf_engine.h")
 
 def header_functions():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(letkf_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(letkf_\w+)\(", txt, re.M)))
 
 
 def test_exports_every_declared_symbol():
@@ -41,6 +41,9 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", sizeof(letkf_params_t), sizeof(letkf_grid_t), sizeof(letkf_obs_t), sizeof(letkf_stats_t));
   printf("%zu %zu %zu\n", offsetof(letkf_params_t, pi), offsetof(letkf_params_t, smax), offsetof(letkf_params_t, stage_timing));
   printf("%zu %zu\n", offsetof(letkf_obs_t, nobsgrd), offsetof(letkf_stats_t, ms_total));
+  printf("%zu %zu %zu %zu\n", sizeof(letkf_xtoy_grid_t), offsetof(letkf_xtoy_grid_t, hnosea),
+         sizeof(letkf_setobs_cfg_t), offsetof(letkf_setobs_cfg_t, qc_max));
+  printf("%zu %zu\n", sizeof(letkf_obs_set_t), offsetof(letkf_obs_set_t, nobs_out));
   return 0;
 }''')
     exe = tmp_path / "lay"
@@ -49,7 +52,9 @@ int main(void) {
     got = [int(x) for x in out]
     want = [C.sizeof(_lib.Params), C.sizeof(_lib.Grid), C.sizeof(_lib.Obs), C.sizeof(_lib.Stats),
             _lib.Params.pi.offset, _lib.Params.smax.offset, _lib.Params.stage_timing.offset,
-            _lib.Obs.nobsgrd.offset, _lib.Stats.ms_total.offset]
+            _lib.Obs.nobsgrd.offset, _lib.Stats.ms_total.offset,
+            C.sizeof(_lib.XtoyGrid), _lib.XtoyGrid.hnosea.offset, C.sizeof(_lib.SetObsCfg),
+            _lib.SetObsCfg.qc_max.offset, C.sizeof(_lib.ObsSet), _lib.ObsSet.nobs_out.offset]
     assert got == want
 
 
@@ -91,6 +96,19 @@ def test_argument_errors_are_status_codes():
     g, o = _lib.Grid(), _lib.Obs()
     assert L.letkf_da_main(None, C.byref(g), C.byref(o), None, None, None, None) == -1
     assert L.letkf_destroy(None) == 0
+    # every device entry point rejects a NULL handle before any HIP call
+    # (this also checks that the ctypes argument lists match the C arity)
+    cfg, out = _lib.SetObsCfg(), _lib.ObsSet()
+    L.letkf_setobs_default(C.byref(cfg))
+    assert L.letkf_set_obs_device(None, C.byref(cfg), 8, 6, 0, None, *([None] * 11), C.byref(out)) == -1
+    assert L.letkf_bin_obs(None, 8, 6, None, None, 0, None, None, None, None, None, None, None) == -1
+    assert L.letkf_aoei(None, 0, None, None, None) == -1
+    assert L.letkf_core_batched(None, 0, None, None, None, None, None, None, None) == -1
+    assert L.letkf_da_main_device(None, C.byref(g), C.byref(o), None, None, None, None, None) == -1
+    assert L.letkf_da_adjoint_device(None, C.byref(g), C.byref(o), *([None] * 8)) == -1
+    xg = _lib.XtoyGrid(nlon=8, nlat=6, nlev=3, nv3d=4, nv2d=1, real_kind=8)
+    assert L.letkf_trans_xtoy_device(C.byref(xg), 1, *([None] * 6), 2, None, None, None, None, None,
+                                     None) == -1
     with pytest.raises(_lib.EngineError):
         _lib.check(-1)
 
