@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--xtoy", action="store_true",
                     help="time the observation operator H(x) of every member (default workload c1) on the "
                          "analysis arrays, with an oracle check and CPU baseline on a few members")
+    ap.add_argument("--setobs", action="store_true",
+                    help="time set_obs after H(x) (default workload c1): statistics, QC, compaction, cell "
+                         "sort, AOEI over every obs, with a bit-exact oracle check and CPU baseline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise kernel that reads and writes 1 GiB (three launches), the "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -140,6 +143,87 @@ def run_xtoy(args):
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": gbs / PEAK_HBM_GBS, "traffic": None},
            "cpu_baseline": cpu, "defined_fraction": defined}
+    print(json.dumps(out))
+
+
+def run_setobs(args):
+    """set_obs after the observation operator (common_obs.f90:139-402) on a
+    workload's obs: H(x) of every member from the analysis arrays
+    (untimed input), then one step = letkf_set_obs_device over all obs.
+    Roofline: HBM, algorithmic bytes = hx read once and hdxf written once
+    (nobs x nmem values each) plus 13 per-obs values read (elm, idx, idy,
+    qc, lon, lat, lev, dat, err; hx mean/spread/dep/err staged) and 12
+    written per kept obs."""
+    import numpy as np
+    import torch
+    import pyoracle as po
+    from sbpom_letkf_v2_amd import engine, synth
+    wl = args.workload if args.workload != "c2" else "c1"
+    log(f"generating workload {wl} on the GPU")
+    pr = synth.config(wl, device="cuda", real_kind=8)
+    meta = pr.meta
+    nlon, nlat, nlev, K = meta["nlon"], meta["nlat"], meta["nlev"], meta["nmem"]
+    lon = pr.lon1.reshape(nlat, nlon).contiguous()
+    lat = pr.lat1.reshape(nlat, nlon).contiguous()
+    depth = pr.depth1.reshape(nlev, nlat, nlon).contiguous()
+    fsm = torch.ones_like(lon)
+    dlon = float(lon[0, 1] - lon[0, 0])
+    dlat = float(lat[1, 0] - lat[0, 0])
+    # obs in input (file) order: a fixed shuffle of the workload's obs
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    nobs = int(pr.obselm.numel())
+    sh = torch.randperm(nobs, generator=g, device="cuda")
+    elm, olon, olat, olev, err = (a[sh].contiguous() for a in (pr.obselm, pr.obslon, pr.obslat, pr.obslev,
+                                                               pr.obserr))
+    idx = (torch.floor((olon - lon[0, 0]) / dlon) + 1).to(torch.int32).contiguous()
+    idy = (torch.floor((olat - lat[0, 0]) / dlat) + 1).to(torch.int32).contiguous()
+    v2 = pr.fcst2d if meta["nv2d"] > 0 else torch.zeros((1, K, nlat * nlon), dtype=torch.float64, device="cuda")
+    hx, qc = engine.trans_xtoy_device(lon, lat, depth, fsm, elm, idx, idy, olon, olat, olev, pr.fcst3d, v2,
+                                      analysis_layout=True)
+    dat = (hx.mean(0) + err * torch.randn(nobs, generator=g, device="cuda", dtype=torch.float64)).contiguous()
+    eng = engine.Engine(engine.make_params(pr))
+    call = lambda: eng.set_obs_device(nlon, nlat, elm, olon, olat, olev, dat, err, idx, idy, hx, qc=qc)
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        res = call()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nk = res["nobs"]
+    bytes_step = 8.0 * (nobs * K + nk * K + 13 * nobs + 12 * nk)
+    gbs = bytes_step / (ms * 1e-3) / 1e9
+    log(f"  {ms:.3f} ms per call (host sync included), {nobs / (ms * 1e-3):.3e} obs/s, {gbs:.0f} GB/s "
+        f"algorithmic; kept {nk} of {nobs}")
+    cpu = None
+    if not args.no_cpu:
+        host = lambda t: t.cpu().numpy()
+        c = po.setobs_config(K, nlon, nlat)
+        t0 = time.perf_counter()
+        want = po.set_obs(c, host(elm), host(olon), host(olat), host(olev), host(dat), host(err), host(idx),
+                          host(idy), host(hx).T, qc=host(qc))
+        t_cpu = time.perf_counter() - t0
+        ok = want["nobs"] == nk and np.array_equal(host(res["obshdxf"]).T, want["hdxf"]) and all(
+            np.array_equal(host(res["obs" + k]), want[k]) for k in ("elm", "lon", "err", "dep", "hxfsprd")) \
+            and np.array_equal(host(res["nobsgrd"]).T, want["nobsgrd"])
+        cpu = {"value": nobs / t_cpu, "unit": "obs/s", "cores": 1, "kind": "port",
+               "sample": f"all {nobs} obs x {K} members, FP64 C oracle (one thread), {t_cpu:.2f} s"}
+        log(f"  oracle: bit-exact {ok}; cpu {cpu['value']:.3e} obs/s")
+        if not ok:
+            raise SystemExit("set_obs differs from the oracle")
+    eng.close()
+    out = {"metric": "set_obs observations per second", "value": nobs / (ms * 1e-3), "unit": "obs/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{wl}: {nlon}x{nlat} K={K} nobs={nobs} (kept {nk}), set_obs after H(x) of "
+                                  "every member"},
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": gbs / PEAK_HBM_GBS, "traffic": None},
+           "cpu_baseline": cpu}
     print(json.dumps(out))
 
 
@@ -415,6 +499,8 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.setobs:
+        return run_setobs(args)
     if args.xtoy:
         run_xtoy(args)
         return
