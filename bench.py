@@ -64,7 +64,8 @@ def parse():
 
 def run_xtoy(args):
     """H(x) (Trans_XtoY, letkf/run/src/common.f90:391-500) of all members of a
-    workload's ensemble, read in place from the analysis arrays.  One step =
+    workload's ensemble, read in place from the analysis arrays, for the
+    workload's obs in a shuffled (file) order.  One step =
     one call over every (obs, member).  Roofline: HBM, algorithmic bytes
     = per (obs, member) the 4 x 6 column values an Akima obs reads (4 for
     SSH and surface-level obs) plus the hx store; depth columns are shared
@@ -87,7 +88,12 @@ def run_xtoy(args):
     dlat = float(lat[1, 0] - lat[0, 0])
     idx = (torch.floor((pr.obslon - lon[0, 0]) / dlon) + 1).to(torch.int32).contiguous()
     idy = (torch.floor((pr.obslat - lat[0, 0]) / dlat) + 1).to(torch.int32).contiguous()
-    elm, lono, lato, levo = pr.obselm, pr.obslon, pr.obslat, pr.obslev
+    # obs in input (file) order: a fixed shuffle of the workload's sorted obs
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    sh = torch.randperm(int(pr.obselm.numel()), generator=g, device="cuda")
+    elm, lono, lato, levo = (a[sh].contiguous() for a in (pr.obselm, pr.obslon, pr.obslat, pr.obslev))
+    idx, idy = idx[sh].contiguous(), idy[sh].contiguous()
     v2 = pr.fcst2d if nv2 > 0 else torch.zeros((1, K, nlat * nlon), dtype=torch.float64, device="cuda")
     nobs = int(elm.numel())
     call = lambda: engine.trans_xtoy_device(lon, lat, depth, fsm, elm, idx, idy, lono, lato, levo, pr.fcst3d, v2,

@@ -4,11 +4,16 @@
 // step that produces hdxf (common_io.f90:829-831, one call per member there).
 //
 // One thread per (obs, member), obs fastest, so that a wavefront reads the
-// neighbouring obs of one member (obs arrive sorted by grid cell).  Every
+// neighbouring obs of one member.  Obs arrive in file order, scattered over
+// the grid; above a few thousand obs the threads visit them in grid-cell
+// order (a radix sort of the cell index, stable), so that neighbouring
+// threads read the same or adjacent model columns.  Every
 // value follows the reference's operation order in r_size, with the double
 // literals of akima_spline promoted to FP64 as the Fortran promotes them;
 // contraction into FMA is off, so results are bit-identical to the CPU
 // restatement of the same routines.
+#include <hipcub/hipcub.hpp>
+
 #include "letkf_handle.h"
 
 namespace letkf {
@@ -27,6 +32,7 @@ struct XtoyArgs {
   const T *lono, *lato, *levo;
   const T *v3d, *v2d;  // element (i,j,k,v,m) at i + nlon j + nlon nlat k + sv3 v + sm3 m (v2d: no k)
   long long sv3, sm3, sv2, sm2;
+  const int32_t* order;  // visit order of the obs (NULL: input order)
   T* hx;               // (nobs, nmem)
   int32_t* qc;         // 0 where any member's H(x) is undef
 };
@@ -114,7 +120,8 @@ __global__ void __launch_bounds__(256) k_trans_xtoy(XtoyArgs<T> a) {
 #pragma clang fp contract(off)
   const long long tid = blockIdx.x * 256LL + threadIdx.x;
   if (tid >= (long long)a.nobs * a.nmem) return;
-  const int o = (int)(tid % a.nobs), mem = (int)(tid / a.nobs);
+  const int t = (int)(tid % a.nobs), mem = (int)(tid / a.nobs);
+  const int o = a.order ? a.order[t] : t;
   const long long nij = (long long)a.nlon * a.nlat;
   const T undef = a.undef;
   T h = undef;
@@ -160,6 +167,20 @@ __global__ void __launch_bounds__(256) k_trans_xtoy(XtoyArgs<T> a) {
   if (a.qc && h == undef) a.qc[o] = 0;
 }
 
+// Cell index of each obs (cells outside 1..nlon-1 x 1..nlat-1 share the
+// key ncell) and the identity permutation to sort along.
+__global__ void __launch_bounds__(256) k_xtoy_key(int n, int nlon, int nlat, const int32_t* idx, const int32_t* idy,
+                                                  int32_t* key, int32_t* vals) {
+  const int o = blockIdx.x * 256 + threadIdx.x;
+  if (o >= n) return;
+  const int i = idx[o], j = idy[o];
+  const bool in = i >= 1 && i <= nlon - 1 && j >= 1 && j <= nlat - 1;
+  key[o] = in ? (j - 1) * nlon + (i - 1) : nlon * nlat;
+  vals[o] = o;
+}
+
+constexpr int kXtoySortMin = 4096;  // below this the input order is kept
+
 template <typename T>
 int run_xtoy(const letkf_xtoy_grid_t* g, int nobs, const int32_t* elm, const int32_t* idx, const int32_t* idy,
              const void* lono, const void* lato, const void* levo, int nmem, const void* v3d, const void* v2d,
@@ -182,14 +203,51 @@ int run_xtoy(const letkf_xtoy_grid_t* g, int nobs, const int32_t* elm, const int
   a.sv2 = strides && strides[2] ? strides[2] : nij;
   a.sm2 = strides && strides[3] ? strides[3] : nij * g->nv2d;
   a.hx = (T*)hx; a.qc = qc;
+  a.order = nullptr;
+  // visit order: obs sorted by cell (scratch freed after the final sync)
+  void* tmp = nullptr;
+  if (nobs >= kXtoySortMin) {
+    const int ncell = g->nlon * g->nlat;
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) <= ncell) ++bits;
+    size_t cub_bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, cub_bytes, (int32_t*)nullptr, (int32_t*)nullptr,
+                                           (int32_t*)nullptr, (int32_t*)nullptr, nobs, 0, bits, s) != hipSuccess)
+      return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: sort sizing failed");
+    const size_t nb = ((sizeof(int32_t) * (size_t)nobs + 255) / 256) * 256;
+    hipError_t e = hipMalloc(&tmp, 4 * nb + cub_bytes);
+    if (e != hipSuccess) return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: %s", hipGetErrorString(e));
+    int32_t* key = (int32_t*)tmp;
+    int32_t* key2 = (int32_t*)((char*)tmp + nb);
+    int32_t* vals = (int32_t*)((char*)tmp + 2 * nb);
+    int32_t* order = (int32_t*)((char*)tmp + 3 * nb);
+    hipLaunchKernelGGL(k_xtoy_key, dim3((unsigned)((nobs + 255) / 256)), dim3(256), 0, s, nobs, g->nlon, g->nlat,
+                       idx, idy, key, vals);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipcub::DeviceRadixSort::SortPairs((char*)tmp + 4 * nb, cub_bytes, key, key2, vals, order, nobs, 0,
+                                             bits, s);
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(tmp);
+      return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: %s", hipGetErrorString(e));
+    }
+    a.order = order;
+  }
   if (qc) {
     hipLaunchKernelGGL(k_qc_fill<T>, dim3((unsigned)((nobs + 255) / 256)), dim3(256), 0, s, nobs, qc);
-    if (hipGetLastError() != hipSuccess) return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: launch failed");
+    if (hipGetLastError() != hipSuccess) {
+      (void)hipStreamSynchronize(s);
+      if (tmp) (void)hipFree(tmp);
+      return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: launch failed");
+    }
   }
   const long long n = (long long)nobs * nmem;
   hipLaunchKernelGGL(k_trans_xtoy<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipError_t e2 = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = e2;
+  if (tmp) (void)hipFree(tmp);
   if (e != hipSuccess) return set_err(LETKF_ERR_HIP, "letkf_trans_xtoy_device: %s", hipGetErrorString(e));
   return LETKF_OK;
 }
@@ -215,6 +273,7 @@ extern "C" int letkf_trans_xtoy_device(const letkf_xtoy_grid_t* g, int32_t nobs,
   if (g->iv2d_z < 1 || g->iv2d_z > g->nv2d)
     return set_err(LETKF_ERR_ARG, "iv2d_z %d out of 1..%d", g->iv2d_z, g->nv2d);
   if (nobs < 0 || nmem < 1) return set_err(LETKF_ERR_ARG, "nobs %d, nmem %d", nobs, nmem);
+  if ((long long)g->nlon * g->nlat >= INT32_MAX) return set_err(LETKF_ERR_ARG, "letkf_trans_xtoy_device: grid too large");
   if (nobs == 0) return LETKF_OK;
   if (!elm || !idx || !idy || !lono || !lato || !levo || !v3d || !v2d || !hx || !g->lon || !g->lat || !g->depth ||
       !g->fsm)

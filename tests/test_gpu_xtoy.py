@@ -18,8 +18,9 @@ def _say(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("dtype,nlev,nmem,nobs,seed", [(np.float64, 16, 5, 4000, 1), (np.float32, 16, 5, 4000, 2),
-                                                       (np.float64, 6, 3, 1500, 3), (np.float32, 40, 8, 3000, 4)])
+# nobs >= 4096 runs the obs in grid-cell order, smaller sets in input order
+@pytest.mark.parametrize("dtype,nlev,nmem,nobs,seed", [(np.float64, 16, 5, 9000, 1), (np.float32, 16, 5, 4000, 2),
+                                                       (np.float64, 6, 3, 1500, 3), (np.float32, 40, 8, 6000, 4)])
 def test_trans_xtoy_gpu_vs_oracle(dtype, nlev, nmem, nobs, seed):
     _say(f"case xtoy dtype={np.dtype(dtype).name} nlev={nlev} nmem={nmem} nobs={nobs}")
     rng = np.random.default_rng(seed)
