@@ -157,6 +157,40 @@ int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* grid,
                          const void* fcst2d, void* anal3d, void* anal2d,
                          void* stream);
 
+/* letkf_da_main_device in two parts, for callers that repeat an analysis
+ * over one observation geometry (inflation or error tuning loops, the
+ * adjoint's outer iterations) or capture it in a hipGraph.  No counterpart
+ * in the reference, whose da_main (common_da.f90:22-124) recounts the local
+ * obs on every call.
+ *
+ * letkf_da_main_plan: the geometry-dependent part -- local-obs counts and
+ * offsets of every point, the chunk plan, the size buckets and the
+ * workspaces -- from grid and from obs->nobsgrd/lon/lat/lev (and elm, err
+ * for the search).  It reads results back to the host and may allocate;
+ * it returns after the work on `stream` (NULL: the handle's) is done.
+ *
+ * letkf_da_main_enqueue: the rest of the analysis on the plan (transpose
+ * of hdxf, local-obs fill, Gram, eigensolves, transform), as kernel
+ * launches on `stream` only -- no allocation, no synchronisation, no host
+ * read-back, no memset -- so it may be captured in a hipGraph.  Returns
+ * after enqueueing.  grid, obs and the handle's sizes, form and
+ * localisation (pi, re, sigma_obs(v), dist_zero(v)) must be those of the
+ * plan (the array pointers are compared); the values of fcst, obs->dep,
+ * obs->err, obs->hdxf and the inflation/precision parameters may differ.
+ * The plan and the workspaces a captured graph refers to stay valid until
+ * the next letkf_da_main_plan / letkf_da_main_device / letkf_destroy on the
+ * handle.  Kernel status bits stay on the device.
+ *
+ * letkf_da_main_status: waits for `stream` and returns the status of the
+ * analyses enqueued since (each enqueue clears it first). */
+int letkf_da_main_plan(letkf_handle_t* h, const letkf_grid_t* grid,
+                       const letkf_obs_t* obs, void* stream);
+int letkf_da_main_enqueue(letkf_handle_t* h, const letkf_grid_t* grid,
+                          const letkf_obs_t* obs, const void* fcst3d,
+                          const void* fcst2d, void* anal3d, void* anal2d,
+                          void* stream);
+int letkf_da_main_status(letkf_handle_t* h, void* stream);
+
 /* letkf_core (common_letkf.f90:37-210) batched over nb local problems given
  * in CSR form, DEVICE arrays: problem b owns rows offs[b]..offs[b+1]-1 of
  * hdxf (rows of nmem values, row-major), rdiag, rloc, dep; outputs

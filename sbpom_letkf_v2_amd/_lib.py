@@ -24,7 +24,8 @@ EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_da_main", "letkf_da_main_device", "letkf_core_batched",
            "letkf_bin_obs", "letkf_aoei", "letkf_da_adjoint_device",
            "letkf_set_params", "letkf_get_stats", "letkf_last_error", "letkf_trans_xtoy_device",
-           "letkf_setobs_default", "letkf_set_obs_device")
+           "letkf_setobs_default", "letkf_set_obs_device", "letkf_da_main_plan", "letkf_da_main_enqueue",
+           "letkf_da_main_status")
 
 
 class Params(C.Structure):
@@ -113,6 +114,9 @@ def lib():
                                         vp, vp, vp, vp, vp, vp]
         if hasattr(L, "letkf_aoei"):
             L.letkf_aoei.argtypes = [vp, C.c_int32, vp, vp, vp]
+        L.letkf_da_main_plan.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp]
+        L.letkf_da_main_enqueue.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp, vp]
+        L.letkf_da_main_status.argtypes = [vp, vp]
         L.letkf_da_adjoint_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp,
                                               vp, vp, vp, vp]
         L.letkf_set_params.argtypes = [vp, C.POINTER(Params)]

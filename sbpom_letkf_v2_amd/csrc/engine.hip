@@ -272,33 +272,7 @@ __global__ void __launch_bounds__(256) k_bucket_scatter(long long P, const int32
 }
 
 template <typename T>
-static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2, T* a3,
-                      T* a2, hipStream_t s) {
-  const letkf_params_t& p = h->prm;
-  const int K = p.nmem;
-  KParams kp = make_kparams(p, o->nobs);
-  const long long P = kp.P;
-  const size_t n3 = (size_t)P * K * p.nv3d, n2 = (size_t)p.nij1 * K * p.nv2d;
-  letkf_stats_t st{};
-  st.points = P;
-  hipEvent_t ev0, ev1;
-  HIP_TRY(hipEventCreate(&ev0));
-  HIP_TRY(hipEventCreate(&ev1));
-  HIP_TRY(hipEventRecord(ev0, s));
-  if (o->nobs == 0) {
-    // common_da.f90:57-61: analysis = forecast, no inflation, no round_off
-    HIP_TRY(launch_copy<T>(f3, a3, n3, s));
-    HIP_TRY(launch_copy<T>(f2, a2, n2, s));
-    HIP_TRY(hipEventRecord(ev1, s));
-    HIP_TRY(hipEventSynchronize(ev1));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    st.ms_total = ms;
-    h->stats = st;
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    return LETKF_OK;
-  }
+static Tables<T> make_tables(const letkf_grid_t* g, const letkf_obs_t* o, const double2* osc) {
   Tables<T> tb;
   tb.lon1 = (const T*)g->lon1;
   tb.lat1 = (const T*)g->lat1;
@@ -315,31 +289,71 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   tb.olev = (const T*)o->lev;
   tb.oerr = (const T*)o->err;
   tb.odep = (const T*)o->dep;
-  HIP_TRY(h->otrig.ensure(sizeof(double2) * (size_t)(o->nobs + 1)));
-  tb.osc = h->otrig.as<double2>();
-  HIP_TRY(launch_obs_trig<T>(o->nobs, p.pi, tb.olat, h->otrig.as<double2>(), s));
+  tb.osc = osc;
+  return tb;
+}
 
+__global__ void k_zero_status(int* st) {
+  if (threadIdx.x < 4) st[threadIdx.x] = 0;
+}
+
+// Parameters an analysis plan depends on: sizes, form and localisation.
+static bool same_geometry_params(const letkf_params_t& a, const letkf_params_t& b) {
+  return a.nmem == b.nmem && a.nlon == b.nlon && a.nlat == b.nlat && a.nlev == b.nlev && a.nij1 == b.nij1 &&
+         a.nv3d == b.nv3d && a.nv2d == b.nv2d && a.real_kind == b.real_kind && a.form == b.form &&
+         a.id_z_obs == b.id_z_obs && a.pi == b.pi && a.re == b.re && a.sigma_obs == b.sigma_obs &&
+         a.sigma_obsv == b.sigma_obsv && a.dist_zero == b.dist_zero && a.dist_zerov == b.dist_zerov;
+}
+
+static void plan_key(const letkf_grid_t* g, const letkf_obs_t* o, const void* key[12]) {
+  const void* k[12] = {g->lon1, g->lat1, g->ri1, g->rj1, g->dlon1, g->dlat1, g->phi1, g->depth1,
+                       o->nobsgrd, o->lon, o->lat, o->lev};
+  for (int q = 0; q < 12; ++q) key[q] = k[q];
+}
+
+// The part of an analysis fixed by the geometry (grid, obs positions and
+// levels, localisation scales): local-obs counts and offsets of every
+// point, the chunk plan, the size buckets of the active slots and the
+// workspaces.  Host read-backs and allocations happen here only.
+template <typename T>
+static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, hipStream_t s, bool timing) {
+  AnalysisPlan& pl = h->plan;
+  pl.valid = false;
+  const letkf_params_t& p = h->prm;
+  const int K = p.nmem;
+  KParams kp = make_kparams(p, o->nobs);
+  const long long P = kp.P;
+  letkf_stats_t st{};
+  st.points = P;
+  pl.prm = p;
+  pl.nobs = o->nobs;
+  plan_key(g, o, pl.key);
+  pl.P = P;
+  if (o->nobs == 0) {
+    pl.nch = 0;
+    pl.st = st;
+    pl.valid = true;
+    return LETKF_OK;
+  }
+  HIP_TRY(h->otrig.ensure(sizeof(double2) * (size_t)(o->nobs + 1)));
+  Tables<T> tb = make_tables<T>(g, o, h->otrig.as<double2>());
+  HIP_TRY(launch_obs_trig<T>(o->nobs, p.pi, tb.olat, h->otrig.as<double2>(), s));
   HIP_TRY(h->nloc.ensure(sizeof(int32_t) * (P + 1)));
   HIP_TRY(h->loff.ensure(sizeof(int64_t) * (P + 1)));
   HIP_TRY(h->actidx.ensure(sizeof(int32_t) * (P + 1)));
   HIP_TRY(h->act_p.ensure(sizeof(int32_t) * (P + 1)));
   HIP_TRY(h->scan_tmp.ensure(scan_tmp_bytes(P)));
   HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
-  HIP_TRY(hipMemsetAsync(h->status.ptr, 0, sizeof(int) * 4, s));
-
-  const bool timing = p.stage_timing != 0;
   const int dual = p.form == LETKF_FORM_BATCHED;
-  // stage marks: [0,1] search count+scan, then per chunk 5 marks
-  auto mark = [&](size_t q) -> hipError_t { return timing ? hipEventRecord(h->events[q], s) : hipSuccess; };
-  // obs-major FP64 copy of obshdxf
-  HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
   // count_nobsl for every point, then the scans
-  if (timing) HIP_TRY(h->ensure_events(2));
-  HIP_TRY(mark(0));
+  if (timing) {
+    HIP_TRY(h->ensure_events(2));
+    HIP_TRY(hipEventRecord(h->events[0], s));
+  }
   HIP_TRY(launch_search_count<T>(kp, tb, h->nloc.as<int32_t>(), s));
   HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
                              h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
-  HIP_TRY(mark(1));
+  if (timing) HIP_TRY(hipEventRecord(h->events[1], s));
 
   // Chunk of consecutive points: its active slots own a K x K Gram and
   // eigenvector block and the eigensolve's rotation log.  The register
@@ -366,7 +380,8 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   }
   PC = std::max(1024LL, std::min(PC, P));
   const long long nch = (P + PC - 1) / PC;
-  std::vector<long long> bpts(nch + 1);
+  std::vector<long long>& bpts = pl.bpts;
+  bpts.assign(nch + 1, 0);
   for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
   HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 6)));
   long long* d_pts = h->bounds.as<long long>();
@@ -380,7 +395,8 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   hipLaunchKernelGGL(k_gather_bounds, dim3((unsigned)((nch + 1 + 255) / 256)), dim3(256), 0, s, (int)(nch + 1),
                      d_pts, h->loff.as<int64_t>(), h->actidx.as<int32_t>(), d_out);
   HIP_TRY(hipGetLastError());
-  std::vector<long long> bv(2 * (nch + 1) + 6);
+  std::vector<long long>& bv = pl.bv;
+  bv.assign(2 * (nch + 1) + 6, 0);
   HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * (2 * (nch + 1) + 6), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   st.max_nobsl = bv[2 * (nch + 1)];
@@ -395,7 +411,10 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   // register eigensolve for K <= 128: bucket the active slots of every chunk
   // by problem size; bucket (c, q) occupies list[boff[c*kNQ+q] ...)
   const bool reg_eig = K <= 128;
-  std::vector<int32_t> hist, boff;
+  std::vector<int32_t>& hist = pl.hist;
+  std::vector<int32_t>& boff = pl.boff;
+  hist.clear();
+  boff.clear();
   if (reg_eig) {
     const size_t nh = (size_t)nch * kNQ;
     HIP_TRY(h->bhist.ensure(sizeof(int32_t) * nh));
@@ -442,18 +461,60 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     HIP_TRY(h->logx.ensure((size_t)(maxA + 1) * eigr_ix_pitch(K)));
     HIP_TRY(h->logr.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
   }
-
-  if (timing) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
-  // per-kernel marks of the register eigensolve: 5 kernels per bucket launch
-  EigTimer ktm{nullptr, nullptr, 0, 0, nullptr};
-  if (p.stage_timing >= 2 && reg_eig) {
-    const int cap = (int)(5 * kNQ * nch);
-    HIP_TRY(h->ensure_kevents(2 * (size_t)cap));
-    h->ktags.assign(cap, 0);
-    HIP_TRY(h->rotsum.ensure(sizeof(unsigned long long)));
-    HIP_TRY(hipMemsetAsync(h->rotsum.ptr, 0, sizeof(unsigned long long), s));
-    ktm = EigTimer{h->kevents.data(), h->ktags.data(), cap, 0, h->rotsum.as<unsigned long long>()};
+  st.chunks = nch;
+  st.gram_flops = 2.0 * K * ((double)bv[2 * (nch + 1) + 2] + (double)K * (double)bv[2 * (nch + 1) + 3]);
+  st.eig_flops = 4.0 / 3.0 * (double)bv[2 * (nch + 1) + 4];
+  // the tile transform reads and writes the ensemble of the points it
+  // updates: every point on the LDS path, only points without local obs
+  // (and the rare overflowed eigensolves, not counted) on the register path
+  if (reg_eig) {
+    const double noobs = (double)(P - st.active_points), noobs_surf = (double)bv[2 * (nch + 1) + 5];
+    st.transform_bytes = 2.0 * sizeof(T) * K * (noobs * p.nv3d + noobs_surf * p.nv2d);
+  } else {
+    const double n3 = (double)P * K * p.nv3d, n2 = (double)p.nij1 * K * p.nv2d;
+    st.transform_bytes = 2.0 * sizeof(T) * (n3 + n2);
   }
+  pl.PC = PC;
+  pl.nch = nch;
+  pl.yp = yp;
+  pl.reg_eig = reg_eig;
+  pl.dual = dual;
+  pl.st = st;
+  pl.valid = true;
+  return LETKF_OK;
+}
+
+// The part of an analysis that depends on the values (forecast ensemble,
+// innovations, obs errors and perturbations, inflation): only kernel
+// launches on `s` -- no allocation, no host synchronisation, no read-back,
+// no memset -- so it can be captured in a hipGraph.  Status bits stay in
+// the handle's device word.  timing (not under capture): stage events, and
+// per-kernel events of the eigensolve when ktm is given.
+template <typename T>
+static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2,
+                          T* a3, T* a2, hipStream_t s, bool timing, EigTimer* ktm) {
+  const AnalysisPlan& pl = h->plan;
+  const letkf_params_t& p = h->prm;
+  const int K = p.nmem;
+  KParams kp = make_kparams(p, o->nobs);
+  const long long P = kp.P;
+  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, h->status.as<int>());
+  HIP_TRY(hipGetLastError());
+  if (o->nobs == 0) {
+    // common_da.f90:57-61: analysis = forecast, no inflation, no round_off
+    HIP_TRY(launch_copy<T>(f3, a3, (size_t)P * K * p.nv3d, s));
+    HIP_TRY(launch_copy<T>(f2, a2, (size_t)p.nij1 * K * p.nv2d, s));
+    return LETKF_OK;
+  }
+  Tables<T> tb = make_tables<T>(g, o, h->otrig.as<double2>());
+  const int dual = pl.dual;
+  const bool reg_eig = pl.reg_eig;
+  const long long nch = pl.nch, yp = pl.yp;
+  const std::vector<long long>& bpts = pl.bpts;
+  const std::vector<long long>& bv = pl.bv;
+  auto mark = [&](size_t q) -> hipError_t { return timing ? hipEventRecord(h->events[q], s) : hipSuccess; };
+  // obs-major FP64 copy of obshdxf
+  HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
   for (long long c = 0; c < nch; ++c) {
     const size_t e0 = 2 + 5 * (size_t)c;
     const long long p0 = bpts[c], p1 = bpts[c + 1];
@@ -472,25 +533,25 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
                                      h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0,
                                      na, h->G.as<double>(), h->bvec.as<double>(), dual, s));
       else
-      HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
-                                  lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na, h->G.as<double>(),
-                                  h->bvec.as<double>(), dual, s));
+        HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
+                                    lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
+                                    h->G.as<double>(), h->bvec.as<double>(), dual, s));
     }
     HIP_TRY(mark(e0 + 2));
     if (na > 0) {
       if (reg_eig) {
         for (int q = 0; q < kNQ; ++q) {
-          const int cnt = hist[(size_t)c * kNQ + q];
+          const int cnt = pl.hist[(size_t)c * kNQ + q];
           if (cnt == 0) continue;
           HIP_TRY(launch_eigr_update<T>(bucket_nb(q), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
-                                        h->blist.as<int32_t>() + boff[(size_t)c * kNQ + q], cnt, h->G.as<double>(),
-                                        h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
-                                        h->logc.as<double>(), h->logx.as<uint8_t>(), h->logi.as<int32_t>(),
-                                        h->logn.as<int32_t>(), h->logr.as<int32_t>(), h->lam.as<double>(),
-                                        h->V.as<double>(), h->yvec.as<double>(), yp, kp, tb, f3, f2, a3, a2,
-                                        h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
-                                        h->yT.as<double>(), h->bvec.as<double>(), h->status.as<int>(),
-                                        ktm.ev ? &ktm : nullptr, s));
+                                        h->blist.as<int32_t>() + pl.boff[(size_t)c * kNQ + q], cnt,
+                                        h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(),
+                                        h->eE.as<double>(), h->logc.as<double>(), h->logx.as<uint8_t>(),
+                                        h->logi.as<int32_t>(), h->logn.as<int32_t>(), h->logr.as<int32_t>(),
+                                        h->lam.as<double>(), h->V.as<double>(), h->yvec.as<double>(), yp, kp, tb, f3,
+                                        f2, a3, a2, h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
+                                        h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
+                                        h->status.as<int>(), ktm, s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -507,10 +568,41 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
                                 h->status.as<int>() + 1, s));
     HIP_TRY(mark(e0 + 4));
   }
+  return LETKF_OK;
+}
+
+// One analysis: plan, enqueue, wait, status and stats.
+template <typename T>
+static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2, T* a3,
+                      T* a2, hipStream_t s) {
+  const letkf_params_t& p = h->prm;
+  const bool timing = p.stage_timing != 0;
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  HIP_TRY(hipEventRecord(ev0, s));
+  int rc = plan_device<T>(h, g, o, s, timing && o->nobs > 0);
+  if (rc) return rc;
+  const AnalysisPlan& pl = h->plan;
+  const long long nch = pl.nch;
+  if (timing && o->nobs > 0) HIP_TRY(h->ensure_events(2 + 5 * (size_t)nch));
+  // per-kernel marks of the register eigensolve: 5 kernels per bucket launch
+  EigTimer ktm{nullptr, nullptr, 0, 0, nullptr};
+  if (p.stage_timing >= 2 && pl.reg_eig && o->nobs > 0) {
+    const int cap = (int)(5 * kNQ * nch);
+    HIP_TRY(h->ensure_kevents(2 * (size_t)cap));
+    h->ktags.assign(cap, 0);
+    HIP_TRY(h->rotsum.ensure(sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(h->rotsum.ptr, 0, sizeof(unsigned long long), s));
+    ktm = EigTimer{h->kevents.data(), h->ktags.data(), cap, 0, h->rotsum.as<unsigned long long>()};
+  }
+  rc = enqueue_device<T>(h, g, o, f3, f2, a3, a2, s, timing && o->nobs > 0, ktm.ev ? &ktm : nullptr);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(ev1, s));
   int stat4[4] = {0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(stat4, h->status.ptr, sizeof(stat4), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  letkf_stats_t st = pl.st;
   const int status = stat4[0];
   st.fallback_slots = stat4[2];
   if (ktm.rot) {
@@ -523,7 +615,7 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   st.ms_total = ms;
-  if (timing) {
+  if (timing && o->nobs > 0) {
     float t = 0.f;
     HIP_TRY(hipEventElapsedTime(&t, h->events[0], h->events[1]));
     st.ms_search = t;
@@ -541,18 +633,6 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
     HIP_TRY(hipEventElapsedTime(&t, ktm.ev[2 * u], ktm.ev[2 * u + 1]));
     st.ms_eigk[ktm.tag[u]] += t;
     st.n_eigk[ktm.tag[u]] += 1;
-  }
-  st.chunks = nch;
-  st.gram_flops = 2.0 * K * ((double)bv[2 * (nch + 1) + 2] + (double)K * (double)bv[2 * (nch + 1) + 3]);
-  st.eig_flops = 4.0 / 3.0 * (double)bv[2 * (nch + 1) + 4];
-  // the tile transform reads and writes the ensemble of the points it
-  // updates: every point on the LDS path, only points without local obs
-  // (and the rare overflowed eigensolves, not counted) on the register path
-  if (reg_eig) {
-    const double noobs = (double)(P - st.active_points), noobs_surf = (double)bv[2 * (nch + 1) + 5];
-    st.transform_bytes = 2.0 * sizeof(T) * K * (noobs * p.nv3d + noobs_surf * p.nv2d);
-  } else {
-    st.transform_bytes = 2.0 * sizeof(T) * ((double)n3 + (double)n2);
   }
   h->stats = st;
   return status_to_rc(status);
@@ -683,6 +763,49 @@ extern "C" int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* g, co
   if (h->prm.real_kind == 8)
     return run_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s);
   return run_device<float>(h, g, o, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s);
+}
+
+extern "C" int letkf_da_main_plan(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, void* stream) {
+  int rc = check_call(h, g, o, (const void*)1, (const void*)1, (void*)1, (void*)1);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  rc = h->prm.real_kind == 8 ? plan_device<double>(h, g, o, s, false) : plan_device<float>(h, g, o, s, false);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  h->stats = h->plan.st;
+  return LETKF_OK;
+}
+
+extern "C" int letkf_da_main_enqueue(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o,
+                                     const void* f3, const void* f2, void* a3, void* a2, void* stream) {
+  int rc = check_call(h, g, o, f3, f2, a3, a2);
+  if (rc) return rc;
+  const AnalysisPlan& pl = h->plan;
+  if (!pl.valid) return set_err(LETKF_ERR_ARG, "letkf_da_main_enqueue: no plan (call letkf_da_main_plan first)");
+  const void* key[12];
+  plan_key(g, o, key);
+  bool same = o->nobs == pl.nobs && same_geometry_params(h->prm, pl.prm);
+  for (int q = 0; q < 12 && same; ++q) same = key[q] == pl.key[q];
+  if (!same)
+    return set_err(LETKF_ERR_ARG, "letkf_da_main_enqueue: grid, obs positions or localisation differ from the plan");
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  if (h->prm.real_kind == 8)
+    return enqueue_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s, false,
+                                  nullptr);
+  return enqueue_device<float>(h, g, o, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s, false,
+                               nullptr);
+}
+
+extern "C" int letkf_da_main_status(letkf_handle_t* h, void* stream) {
+  if (!h) return set_err(LETKF_ERR_ARG, "handle is NULL");
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  int stat4[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(stat4, h->status.ptr, sizeof(stat4), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return status_to_rc(stat4[0]);
 }
 
 extern "C" int letkf_da_adjoint_device(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o,

@@ -47,8 +47,25 @@ struct DevBuf {
   }
 };
 
+// The geometry-dependent part of an analysis (local-obs counts, chunk plan,
+// size buckets), made by letkf_da_main_plan or inside letkf_da_main_device
+// and reused by letkf_da_main_enqueue.
+struct AnalysisPlan {
+  bool valid = false;
+  letkf_params_t prm{};     // parameters it was made with
+  const void* key[12] = {};  // grid and obs position arrays it was made from
+  int nobs = 0;
+  long long P = 0, PC = 0, nch = 0, yp = 0;
+  bool reg_eig = false;
+  int dual = 0;
+  std::vector<long long> bpts, bv;  // chunk point bounds; per-chunk (local obs, active slot) offsets + totals
+  std::vector<int32_t> hist, boff;  // per (chunk, size bucket) slot counts and list offsets
+  letkf_stats_t st{};               // counters known from the plan
+};
+
 struct letkf_handle {
   letkf_params_t prm;
+  AnalysisPlan plan;
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf nloc, loff, actidx, act_p, scan_tmp, yT, lobs, lrloc, G, lam, V, bvec, status, bounds;

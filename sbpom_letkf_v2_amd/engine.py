@@ -151,6 +151,24 @@ class Engine:
                                               ptr(pr.fcst2d), ptr(anal3d), ptr(anal2d), s))
         return anal3d, anal2d
 
+    def plan_device(self, pr, stream=None):
+        """letkf_da_main_plan: the geometry-dependent part of the analysis of
+        pr (local-obs counts, chunk plan, size buckets, workspaces)."""
+        g, o = self._device_structs(pr)
+        check(_lib.lib().letkf_da_main_plan(self.h, C.byref(g), C.byref(o), C.c_void_p(stream) if stream else None))
+
+    def enqueue_device(self, pr, anal3d, anal2d, stream=None):
+        """letkf_da_main_enqueue: the analysis of pr on the current plan as
+        kernel launches on `stream` only (capturable in a graph).  pr's
+        arrays must be the planned ones; their values may change."""
+        g, o = self._device_structs(pr)
+        check(_lib.lib().letkf_da_main_enqueue(self.h, C.byref(g), C.byref(o), _ptr(pr.fcst3d), _ptr(pr.fcst2d),
+                                               _ptr(anal3d), _ptr(anal2d), C.c_void_p(stream) if stream else None))
+
+    def status(self, stream=None):
+        """letkf_da_main_status: wait for `stream`, raise on kernel status bits."""
+        check(_lib.lib().letkf_da_main_status(self.h, C.c_void_p(stream) if stream else None))
+
     def set_obs_device(self, nlon, nlat, elm, lon, lat, lev, dat, err, idx, idy, hx, qc=None, ins=None,
                        nobslots=None, **cfg):
         """letkf_set_obs_device: set_obs after H(x) (common_obs.f90:139-402)
