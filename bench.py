@@ -46,6 +46,8 @@ def parse():
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the plan-once + graph-replay timing of the chosen form")
     ap.add_argument("--adjoint", action="store_true",
                     help="differentiable workload (default c3): time forward + hand-written adjoint and "
                          "check it by central finite differences")
@@ -420,6 +422,38 @@ def run_form(prd, precision, form, steps, warmup):
     return a3, a2, wall / steps, stage, st
 
 
+def graph_replay(prd, precision, form, steps, a3_eager, a2_eager):
+    """The same analysis planned once (letkf_da_main_plan), its enqueue
+    captured in a graph and replayed: ms per replay, and whether a replay
+    reproduces the eager analysis bit for bit."""
+    import torch
+    from sbpom_letkf_v2_amd import engine
+    eng = engine.Engine(engine.make_params(prd, precision, form))
+    try:
+        t = time.perf_counter()
+        eng.plan_device(prd)
+        plan_ms = (time.perf_counter() - t) * 1e3
+        a3, a2 = torch.empty_like(prd.fcst3d), torch.empty_like(prd.fcst2d)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, capture_error_mode="thread_local"):
+            eng.enqueue_device(prd, a3, a2, stream=torch.cuda.current_stream().cuda_stream)
+        g.replay()
+        torch.cuda.synchronize()
+        eng.status()
+        same = bool(torch.equal(a3, a3_eager) and torch.equal(a2, a2_eager))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        eng.status()
+        ms = e0.elapsed_time(e1) / steps
+    finally:
+        eng.close()
+    return {"ms_per_replay": ms, "plan_ms": plan_ms, "bit_exact_vs_eager": same}
+
+
 def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2):
     """FP64 oracle (the C restatement, OpenMP) on a uniform random sample of
     points of the same workload; also returns the GPU-vs-oracle error at the
@@ -544,6 +578,10 @@ def main():
         log(f"  speedup {psec / sec:.2f}x; errors {result_extra['errors_vs_fp64']['summary']}")
     elif args.precision == "fp64":
         ref3, ref2 = a3, a2
+    if not args.no_graph:
+        log("chosen form planned once, enqueue captured in a graph, replayed")
+        result_extra["graph_replay"] = graph_replay(prd, args.precision, args.form, args.steps, a3, a2)
+        log(f"  {result_extra['graph_replay']}")
     # --- roofline of the dominant kernel: HIP events around its launches in the timed steps ---
     L = st["local_obs"]
     na = st["active_points"]

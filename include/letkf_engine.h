@@ -191,6 +191,26 @@ int letkf_da_main_enqueue(letkf_handle_t* h, const letkf_grid_t* grid,
                           void* stream);
 int letkf_da_main_status(letkf_handle_t* h, void* stream);
 
+/* In-project autotuner (no counterpart in the reference).  The register
+ * eigensolve groups the active points of a chunk by local-problem size into
+ * buckets of at most 16, 32, 48, 64 and 128; for nmem <= 64 each bucket has
+ * two vector paths: 1 = the orthogonal factor Q of the tridiagonalisation
+ * accumulated and applied as a matrix, 2 = its Householder reflectors
+ * applied to the vectors one by one.  letkf_autotune times both on the
+ * first chunk of the given analysis (DEVICE arrays, as
+ * letkf_da_main_device), keeps the faster per bucket in the handle, then
+ * runs the whole analysis with them (anal3d/anal2d receive it).
+ * ms (may be NULL) receives 10 times: ms[2q + v - 1] for bucket q (16, 32,
+ * 48, 64, 128) under path v, -1 where not measured.  letkf_set_tuning /
+ * letkf_get_tuning set or read the five choices (0 = the fixed rule:
+ * reflectors for buckets 64 and 128 or nmem > 64, Q below). */
+int letkf_autotune(letkf_handle_t* h, const letkf_grid_t* grid,
+                   const letkf_obs_t* obs, const void* fcst3d,
+                   const void* fcst2d, void* anal3d, void* anal2d, double* ms,
+                   void* stream);
+int letkf_set_tuning(letkf_handle_t* h, const int32_t* vpath);
+int letkf_get_tuning(const letkf_handle_t* h, int32_t* vpath);
+
 /* letkf_core (common_letkf.f90:37-210) batched over nb local problems given
  * in CSR form, DEVICE arrays: problem b owns rows offs[b]..offs[b+1]-1 of
  * hdxf (rows of nmem values, row-major), rdiag, rloc, dep; outputs

@@ -25,7 +25,7 @@ EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_bin_obs", "letkf_aoei", "letkf_da_adjoint_device",
            "letkf_set_params", "letkf_get_stats", "letkf_last_error", "letkf_trans_xtoy_device",
            "letkf_setobs_default", "letkf_set_obs_device", "letkf_da_main_plan", "letkf_da_main_enqueue",
-           "letkf_da_main_status")
+           "letkf_da_main_status", "letkf_autotune", "letkf_set_tuning", "letkf_get_tuning")
 
 
 class Params(C.Structure):
@@ -117,6 +117,10 @@ def lib():
         L.letkf_da_main_plan.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp]
         L.letkf_da_main_enqueue.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp, vp]
         L.letkf_da_main_status.argtypes = [vp, vp]
+        L.letkf_autotune.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp,
+                                     C.POINTER(C.c_double), vp]
+        L.letkf_set_tuning.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.letkf_get_tuning.argtypes = [vp, C.POINTER(C.c_int32)]
         L.letkf_da_adjoint_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), vp, vp, vp, vp,
                                               vp, vp, vp, vp]
         L.letkf_set_params.argtypes = [vp, C.POINTER(Params)]

@@ -551,7 +551,7 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
                                         h->lam.as<double>(), h->V.as<double>(), h->yvec.as<double>(), yp, kp, tb, f3,
                                         f2, a3, a2, h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
                                         h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
-                                        h->status.as<int>(), ktm, s));
+                                        h->status.as<int>(), h->vpath[q], ktm, s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -763,6 +763,105 @@ extern "C" int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* g, co
   if (h->prm.real_kind == 8)
     return run_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s);
   return run_device<float>(h, g, o, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s);
+}
+
+// In-project autotuner of the register eigensolve's vector path: on the
+// first chunk of this analysis, every size bucket (16..64, K <= 64) runs its
+// eigensolve + update with the Q path and with the reflector path, three
+// timed runs each after a warm-up, and keeps the faster; then the whole
+// analysis runs with the chosen paths.  ms[2q + v - 1] receives the best
+// time of bucket q under path v (-1: not measured).
+template <typename T>
+static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2,
+                           T* a3, T* a2, hipStream_t s, double* ms) {
+  for (int q = 0; q < 2 * kNQ; ++q) ms[q] = -1.0;
+  const letkf_params_t& p = h->prm;
+  const int K = p.nmem;
+  if (K <= 64 && o->nobs > 0) {
+    int rc = plan_device<T>(h, g, o, s, false);
+    if (rc) return rc;
+    const AnalysisPlan& pl = h->plan;
+    const int a0 = pl.nch > 0 ? (int)pl.bv[1] : 0;
+    const int na = pl.nch > 0 ? (int)(pl.bv[3] - a0) : 0;
+    if (pl.reg_eig && na > 0) {
+      KParams kp = make_kparams(p, o->nobs);
+      Tables<T> tb = make_tables<T>(g, o, h->otrig.as<double2>());
+      const int dual = pl.dual;
+      const long long lbase = pl.bv[0];
+      hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, h->status.as<int>());
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
+      HIP_TRY(launch_search_fill<T>(kp, tb, pl.bpts[0], pl.bpts[1], h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
+                                    h->lrloc.as<double>(), s));
+      if (p.precision != LETKF_PREC_FP64)
+        HIP_TRY(launch_gram_mixed<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
+                                     h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0,
+                                     na, h->G.as<double>(), h->bvec.as<double>(), dual, s));
+      else
+        HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
+                                    lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
+                                    h->G.as<double>(), h->bvec.as<double>(), dual, s));
+      hipEvent_t e0, e1;
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      for (int q = 0; q < 4; ++q) {
+        const int cnt = pl.hist[q];
+        if (cnt == 0) continue;
+        for (int v = 1; v <= 2; ++v) {
+          float best = 1e30f;
+          for (int r = 0; r < 4; ++r) {  // run 0 warms up
+            HIP_TRY(hipEventRecord(e0, s));
+            HIP_TRY(launch_eigr_update<T>(bucket_nb(q), K, dual, h->nloc.as<int32_t>(), h->act_p.as<int32_t>(), a0,
+                                          h->blist.as<int32_t>() + pl.boff[q], cnt, h->G.as<double>(),
+                                          h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
+                                          h->logc.as<double>(), h->logx.as<uint8_t>(), h->logi.as<int32_t>(),
+                                          h->logn.as<int32_t>(), h->logr.as<int32_t>(), h->lam.as<double>(),
+                                          h->V.as<double>(), h->yvec.as<double>(), pl.yp, kp, tb, f3, f2, a3, a2,
+                                          h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
+                                          h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
+                                          h->status.as<int>(), v, nullptr, s));
+            HIP_TRY(hipEventRecord(e1, s));
+            HIP_TRY(hipEventSynchronize(e1));
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+            if (r > 0 && t < best) best = t;
+          }
+          ms[2 * q + v - 1] = best;
+        }
+        h->vpath[q] = ms[2 * q + 1] < ms[2 * q] ? 2 : 1;
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  }
+  return run_device<T>(h, g, o, f3, f2, a3, a2, s);
+}
+
+extern "C" int letkf_autotune(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
+                              const void* f2, void* a3, void* a2, double* ms, void* stream) {
+  int rc = check_call(h, g, o, f3, f2, a3, a2);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  double loc[2 * kNQ];
+  double* m = ms ? ms : loc;
+  if (h->prm.real_kind == 8)
+    return autotune_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s, m);
+  return autotune_device<float>(h, g, o, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s, m);
+}
+
+extern "C" int letkf_set_tuning(letkf_handle_t* h, const int32_t* vpath) {
+  if (!h || !vpath) return set_err(LETKF_ERR_ARG, "NULL argument");
+  for (int q = 0; q < kNQ; ++q)
+    if (vpath[q] < 0 || vpath[q] > 2) return set_err(LETKF_ERR_ARG, "vector path %d of bucket %d not in 0..2", vpath[q], q);
+  for (int q = 0; q < kNQ; ++q) h->vpath[q] = vpath[q];
+  return LETKF_OK;
+}
+
+extern "C" int letkf_get_tuning(const letkf_handle_t* h, int32_t* vpath) {
+  if (!h || !vpath) return set_err(LETKF_ERR_ARG, "NULL argument");
+  for (int q = 0; q < kNQ; ++q) vpath[q] = h->vpath[q];
+  return LETKF_OK;
 }
 
 extern "C" int letkf_da_main_plan(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, void* stream) {

@@ -66,6 +66,7 @@ struct AnalysisPlan {
 struct letkf_handle {
   letkf_params_t prm;
   AnalysisPlan plan;
+  int vpath[5] = {0, 0, 0, 0, 0};  // vector path per size bucket (16, 32, 48, 64, 128): letkf_autotune
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf nloc, loff, actidx, act_p, scan_tmp, yT, lobs, lrloc, G, lam, V, bvec, status, bounds;

@@ -143,7 +143,8 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
 // 2*eigr_sweep_capacity, count and nrot 1, Y yp >= K*(1+nv3d+nv2d); Q may
 // alias V.  Slots whose rotation log overflowed get lam/V instead
 // (count[slot] == -1, and status[1] set) for launch_transform's fallback
-// mode.
+// mode.  vpath selects the vector path (0 default rule, 1 Q accumulated
+// from the reflectors, K <= 64 only; 2 reflectors applied to the vectors).
 // Events around every eigensolve kernel launch (stage_timing 2): launch u
 // is bracketed by ev[2u], ev[2u+1] and tagged 4*kind + bucket.
 struct EigTimer {
@@ -168,7 +169,7 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const int64_t* loff, long long lbase,
                               const int32_t* lobs, const double* lrloc,
                               const double* yT, const double* bvec,
-                              int* status, EigTimer* tm, hipStream_t s);
+                              int* status, int vpath, EigTimer* tm, hipStream_t s);
 
 // Adjoint of the analysis (FP64, K <= 64): the active slots [0,na) of a
 // chunk (C and b from launch_gram_fp64 in the K x K form) and the points
