@@ -46,6 +46,8 @@ def parse():
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the fixed vector-path rule instead of letkf_autotune on the chosen form")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the plan-once + graph-replay timing of the chosen form")
     ap.add_argument("--adjoint", action="store_true",
@@ -384,7 +386,7 @@ def run_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
-def run_form(prd, precision, form, steps, warmup):
+def run_form(prd, precision, form, steps, warmup, autotune=False):
     import torch
     from sbpom_letkf_v2_amd import engine
     prm = engine.make_params(prd, precision, form)
@@ -392,6 +394,12 @@ def run_form(prd, precision, form, steps, warmup):
     eng = engine.Engine(prm)
     a3 = torch.empty_like(prd.fcst3d)
     a2 = torch.empty_like(prd.fcst2d)
+    tune = None
+    if autotune:
+        t = time.perf_counter()
+        _, _, times, choice = eng.autotune(prd, a3, a2)
+        tune = {"seconds": time.perf_counter() - t, "bucket_ms_Q_vs_reflectors": times, "paths": choice}
+        log(f"  autotune: {tune}")
     for w in range(warmup):
         t = time.perf_counter()
         eng.da_main_device(prd, a3, a2)
@@ -419,10 +427,11 @@ def run_form(prd, precision, form, steps, warmup):
     stage["eig_kernels"] = {EIG_KINDS[q // 5] + f"<{(16, 32, 48, 64, 128)[q % 5]}>": {"ms": kms[q] / steps,
                                                                                     "launches": kn[q] / steps}
                             for q in range(25) if kn[q]}
+    stage["autotune"] = tune
     return a3, a2, wall / steps, stage, st
 
 
-def graph_replay(prd, precision, form, steps, a3_eager, a2_eager):
+def graph_replay(prd, precision, form, steps, a3_eager, a2_eager, paths=None):
     """The same analysis planned once (letkf_da_main_plan), its enqueue
     captured in a graph and replayed: ms per replay, and whether a replay
     reproduces the eager analysis bit for bit."""
@@ -430,6 +439,8 @@ def graph_replay(prd, precision, form, steps, a3_eager, a2_eager):
     from sbpom_letkf_v2_amd import engine
     eng = engine.Engine(engine.make_params(prd, precision, form))
     try:
+        if paths:
+            eng.set_tuning(paths)
         t = time.perf_counter()
         eng.plan_device(prd)
         plan_ms = (time.perf_counter() - t) * 1e3
@@ -564,7 +575,8 @@ def main():
 
     log(f"timing chosen form precision={args.precision} form={args.form}: "
         f"{args.warmup} warmup + {args.steps} steps")
-    a3, a2, sec, stage, st = run_form(prd, args.precision, args.form, args.steps, args.warmup)
+    a3, a2, sec, stage, st = run_form(prd, args.precision, args.form, args.steps, args.warmup,
+                                      autotune=not args.no_autotune)
     log(f"  {sec * 1e3:.1f} ms/step, stages {stage}")
     result_extra = {}
     ref3 = ref2 = None                      # FP64 analysis the errors are measured against
@@ -580,7 +592,8 @@ def main():
         ref3, ref2 = a3, a2
     if not args.no_graph:
         log("chosen form planned once, enqueue captured in a graph, replayed")
-        result_extra["graph_replay"] = graph_replay(prd, args.precision, args.form, args.steps, a3, a2)
+        result_extra["graph_replay"] = graph_replay(prd, args.precision, args.form, args.steps, a3, a2,
+                                                    (stage.get("autotune") or {}).get("paths"))
         log(f"  {result_extra['graph_replay']}")
     # --- roofline of the dominant kernel: HIP events around its launches in the timed steps ---
     L = st["local_obs"]
@@ -631,6 +644,8 @@ def main():
         kernel_roofline[k] = {"ms": v["ms"], "launches": v["launches"], "achieved": a, "unit": v["unit"],
                               "frac": a / v["peak"]}
     stage_ms = {k: v for k, v in stage.items() if k.startswith("ms_")}
+    if stage.get("autotune"):
+        result_extra["autotune"] = stage["autotune"]
     # --- CPU baseline + full-size parity at the sampled points ---
     cpu = None
     parity = None

@@ -198,7 +198,8 @@ int letkf_da_main_status(letkf_handle_t* h, void* stream);
  * accumulated and applied as a matrix, 2 = its Householder reflectors
  * applied to the vectors one by one.  letkf_autotune times both on the
  * first chunk of the given analysis (DEVICE arrays, as
- * letkf_da_main_device), keeps the faster per bucket in the handle, then
+ * letkf_da_main_device), keeps the faster per bucket in the handle (the
+ * fixed rule's path unless the other is more than 3% faster), then
  * runs the whole analysis with them (anal3d/anal2d receive it).
  * ms (may be NULL) receives 10 times: ms[2q + v - 1] for bucket q (16, 32,
  * 48, 64, 128) under path v, -1 where not measured.  letkf_set_tuning /

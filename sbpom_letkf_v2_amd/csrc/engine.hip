@@ -769,8 +769,10 @@ extern "C" int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* g, co
 // first chunk of this analysis, every size bucket (16..64, K <= 64) runs its
 // eigensolve + update with the Q path and with the reflector path, three
 // timed runs each after a warm-up, and keeps the faster; then the whole
-// analysis runs with the chosen paths.  ms[2q + v - 1] receives the best
-// time of bucket q under path v (-1: not measured).
+// analysis runs with the chosen paths.  The fixed rule's path is kept
+// unless the other one is faster by more than 3% (timing noise on a tie
+// must not flip it).  ms[2q + v - 1] receives the best time of bucket q
+// under path v (-1: not measured).
 template <typename T>
 static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2,
                            T* a3, T* a2, hipStream_t s, double* ms) {
@@ -828,7 +830,9 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
           }
           ms[2 * q + v - 1] = best;
         }
-        h->vpath[q] = ms[2 * q + 1] < ms[2 * q] ? 2 : 1;
+        // the fixed rule's path unless the other is faster by more than 3%
+        const int dflt = bucket_nb(q) >= 64 ? 2 : 1, alt = 3 - dflt;
+        h->vpath[q] = ms[2 * q + alt - 1] < 0.97 * ms[2 * q + dflt - 1] ? alt : dflt;
       }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -64,7 +64,9 @@ def test_autotune_choice_reproduces():
     _say(f"  times {times} choice {choice}")
     assert set(times) == {16, 32, 48, 64} and all(choice[nb] in (1, 2) for nb in times)
     for nb, (tq, tr) in times.items():
-        assert choice[nb] == (2 if tr < tq else 1)
+        dflt = 2 if nb >= 64 else 1
+        t_d, t_a = (tr, tq) if dflt == 2 else (tq, tr)
+        assert choice[nb] == ((3 - dflt) if t_a < 0.97 * t_d else dflt)
     fresh = engine.Engine(prm)
     try:
         fresh.set_tuning(choice)
