@@ -176,10 +176,13 @@ int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* grid,
  * after enqueueing.  grid, obs and the handle's sizes, form and
  * localisation (pi, re, sigma_obs(v), dist_zero(v)) must be those of the
  * plan (the array pointers are compared); the values of fcst, obs->dep,
- * obs->err, obs->hdxf and the inflation/precision parameters may differ.
- * The plan and the workspaces a captured graph refers to stay valid until
- * the next letkf_da_main_plan / letkf_da_main_device / letkf_destroy on the
- * handle.  Kernel status bits stay on the device.
+ * obs->err, obs->hdxf and the inflation/precision parameters may differ
+ * between enqueues.  A captured graph replays the kernel arguments of its
+ * capture: new array contents are seen, new parameter values (rho, alpha,
+ * precision) need a new capture.  The plan and the workspaces a captured
+ * graph refers to stay valid until the next letkf_da_main_plan /
+ * letkf_da_main_device / letkf_destroy on the handle.  Kernel status bits
+ * stay on the device.
  *
  * letkf_da_main_status: waits for `stream` and returns the status of the
  * analyses enqueued since (each enqueue clears it first). */

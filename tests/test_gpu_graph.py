@@ -25,6 +25,45 @@ def _problem(real_kind=8, K=24, seed=3):
                               land_frac=0.1, real_kind=real_kind).to("cuda")
 
 
+def test_enqueue_needs_a_plan():
+    _say("case enqueue without a plan")
+    pr = _problem()
+    eng = engine.Engine(engine.make_params(pr, "fp64", "plain"))
+    try:
+        a3, a2 = torch.empty_like(pr.fcst3d), torch.empty_like(pr.fcst2d)
+        with pytest.raises(_lib.EngineError):
+            eng.enqueue_device(pr, a3, a2)
+    finally:
+        eng.close()
+
+
+def test_enqueue_reads_current_parameters():
+    """A new inflation factor between enqueues (same plan) gives the
+    analysis of a fresh handle with that factor."""
+    _say("case enqueue after set_params")
+    pr = _problem()
+    prm = engine.make_params(pr, "fp64", "batched")
+    eng = engine.Engine(prm)
+    try:
+        eng.plan_device(pr)
+        prm2 = engine.make_params(pr, "fp64", "batched")
+        prm2.cov_infl_mul = 1.3
+        prm2.alpha_rtpp = 0.5
+        eng.set_params(prm2)
+        a3, a2 = torch.empty_like(pr.fcst3d), torch.empty_like(pr.fcst2d)
+        eng.enqueue_device(pr, a3, a2)
+        eng.status()
+    finally:
+        eng.close()
+    fresh = engine.Engine(prm2)
+    try:
+        w3, w2 = fresh.da_main_device(pr)
+        torch.cuda.synchronize()
+    finally:
+        fresh.close()
+    assert torch.equal(a3, w3) and torch.equal(a2, w2)
+
+
 @pytest.mark.parametrize("precision,form", [("fp64", "plain"), ("fp32", "batched")])
 def test_plan_enqueue_matches_da_main(precision, form):
     _say(f"case plan+enqueue {precision} {form}")
