@@ -249,14 +249,22 @@ template hipError_t launch_gram_fp64<double>(const KParams&, const Tables<double
 
 // ------------------------------------------------------------------------
 // Mixed-precision Gram on MFMA (K <= 128): the K x K (or dual n x n)
-// product runs as 32x32 output tiles with FP32 accumulation; the four
-// wavefronts of a block split the OUT x OUT output (OUT = 64 or 128) into
-// quadrants of (OUT/64)^2 tiles each, skipping tiles outside the problem:
+// product runs as 32x32 output tiles with FP32 accumulation.  Only the
+// tiles that hold part of the upper triangle are computed (the eigensolvers
+// read no other entry): 3 of the 2x2 tiles of a 64-wide output, one per
+// wavefront; 10 of the 4x4 tiles of a 128-wide output, 3/3/2/2 over the
+// four wavefronts (kTile), tiles past the problem skipped:
 //   prec FP32   : v_mfma_f32_32x32x2f32 on FP32-rounded operands;
 //   prec BF16X3 : operands split x = hi + lo (bf16 each), three products
 //                 hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_bf16.
 // The innovation term b (and dt in the dual form) stays FP64; the result is
-// widened to FP64 for the eigensolve.  Lane layouts (checked on gfx950):
+// widened to FP64 for the eigensolve.  prec FP32_PANEL accumulates in two
+// levels: the MFMA accumulators sum one panel of 32 obs or members and are
+// then added into FP32 running totals, so a rounding error is relative to a
+// panel sum or to the total rather than to a running sum over hundreds of
+// terms (2-6x smaller Gram errors at 400-800 local obs), at the price of
+// 16 more registers per tile (occupancy 8 -> 6 at OUT 64, 4 -> 2 at 128).
+// Lane layouts (checked on gfx950):
 //   32x32x2 f32 : a = A[i][k], i = lane%32, k = lane/32;  b = B[k][j], j = lane%32
 //   32x32x16 bf16: a = A[i][8*(lane/32)+q], b = B[8*(lane/32)+q][j], q = 0..7
 //   C (16 floats): C[(r/4)*8 + (lane/32)*4 + r%4][lane%32]
@@ -265,10 +273,18 @@ typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMfPanel = 32;  // obs (primal) or members (dual) per LDS panel
 
+// Upper-triangle tiles (row tile * 4 + column tile, -1 none) per wavefront.
+__constant__ int kTile64[4][1] = {{0 * 4 + 0}, {0 * 4 + 1}, {1 * 4 + 1}, {-1}};
+__constant__ int kTile128[4][3] = {{0 * 4 + 0, 0 * 4 + 1, 0 * 4 + 2},
+                                   {1 * 4 + 1, 1 * 4 + 2, 0 * 4 + 3},
+                                   {2 * 4 + 2, 1 * 4 + 3, -1},
+                                   {3 * 4 + 3, 2 * 4 + 3, -1}};
+
 template <int PREC, int OUT, typename Src>
 __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, double* __restrict__ G,
                                                    double* __restrict__ bvec, int dual) {
-  constexpr int TW = OUT / 64;   // tiles per wavefront along each dimension
+  constexpr int NT = OUT == 128 ? 3 : 1;          // tiles per wavefront
+  constexpr bool kTwoLevel = PREC == LETKF_PREC_FP32_PANEL;
   constexpr int LP = OUT + 1;    // odd pitch: row-wise and column-wise panel writes hit distinct banks
   constexpr int RS = 256 / OUT;  // primal: threads sharing one output column
   constexpr int kStage = kMfPanel * OUT / 256;  // panel elements staged per thread (8 or 16)
@@ -281,16 +297,27 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   __shared__ int ios[OUT];
   __shared__ double bred[256];
   const int n = src.begin(blockIdx.x);
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int r0 = (wv >> 1) * (OUT / 2), c0 = (wv & 1) * (OUT / 2);  // quadrant of this wave
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: tile tables in scalar registers
   const bool dm = dual && n < K;
   const int N = dm ? n : K;  // output dimension
   const int S = dm ? K : n;  // summation length
-  mf_v16 acc[TW][TW];
+  // this wavefront's tiles: row / column offsets and liveness (wave-uniform)
+  int tr[NT], tc[NT];
+  bool live[NT];
 #pragma unroll
-  for (int a = 0; a < TW; ++a)
+  for (int q = 0; q < NT; ++q) {
+    const int code = OUT == 128 ? kTile128[wv][q] : kTile64[wv][q];
+    tr[q] = code >= 0 ? 32 * (code >> 2) : 0;
+    tc[q] = code >= 0 ? 32 * (code & 3) : 0;
+    live[q] = code >= 0 && tr[q] < N && tc[q] < N;
+  }
+  mf_v16 acc[NT], tot[NT];
 #pragma unroll
-    for (int c = 0; c < TW; ++c) acc[a][c] = mf_v16{};
+  for (int q = 0; q < NT; ++q) {
+    acc[q] = mf_v16{};
+    tot[q] = mf_v16{};
+  }
   if (dm) {
     for (int l = t; l < n; l += 256) {
       int io;
@@ -302,12 +329,6 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
     }
     __syncthreads();
   }
-  // tiles of this wave inside the N x N output
-  bool live[TW][TW];
-#pragma unroll
-  for (int a = 0; a < TW; ++a)
-#pragma unroll
-    for (int c = 0; c < TW; ++c) live[a][c] = r0 + 32 * a < N && c0 + 32 * c < N;
   // primal: b = Y^T diag(rloc/rdiag) dep in FP64, thread t summing column
   // t % OUT over the panel rows it stages
   double bpart = 0.0;
@@ -369,47 +390,45 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
       }
     }
     __syncthreads();
-    if (PREC == LETKF_PREC_FP32) {
+    if (PREC != LETKF_PREC_BF16X3) {
       for (int k = 0; k < ps; k += 2) {
         const int kk = k + (lane >> 5);
-        float av[TW], bv[TW];
 #pragma unroll
-        for (int a = 0; a < TW; ++a) {
-          av[a] = kk < ps ? As[kk][r0 + 32 * a + (lane & 31)] : 0.f;
-          bv[a] = kk < ps ? Bs[kk][c0 + 32 * a + (lane & 31)] : 0.f;
+        for (int q = 0; q < NT; ++q) {
+          if (!live[q]) continue;
+          const float av = kk < ps ? As[kk][tr[q] + (lane & 31)] : 0.f;
+          const float bv = kk < ps ? Bs[kk][tc[q] + (lane & 31)] : 0.f;
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[q], 0, 0, 0);
         }
-#pragma unroll
-        for (int a = 0; a < TW; ++a)
-#pragma unroll
-          for (int c = 0; c < TW; ++c)
-            if (live[a][c]) acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[c], acc[a][c], 0, 0, 0);
       }
     } else {
       for (int k = 0; k < ps; k += 16) {
-        mf_bf8 ah[TW], al[TW], bh[TW], bl[TW];
 #pragma unroll
-        for (int a = 0; a < TW; ++a) {
+        for (int q = 0; q < NT; ++q) {
+          if (!live[q]) continue;
+          mf_bf8 ah, al, bh, bl;
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const int kk = k + 8 * (lane >> 5) + q;
-            const float x = kk < ps ? As[kk][r0 + 32 * a + (lane & 31)] : 0.f;
-            const float y = kk < ps ? Bs[kk][c0 + 32 * a + (lane & 31)] : 0.f;
+          for (int e = 0; e < 8; ++e) {
+            const int kk = k + 8 * (lane >> 5) + e;
+            const float x = kk < ps ? As[kk][tr[q] + (lane & 31)] : 0.f;
+            const float y = kk < ps ? Bs[kk][tc[q] + (lane & 31)] : 0.f;
             const __bf16 x1 = (__bf16)x, y1 = (__bf16)y;
-            ah[a][q] = x1;
-            bh[a][q] = y1;
-            al[a][q] = (__bf16)(x - (float)x1);
-            bl[a][q] = (__bf16)(y - (float)y1);
+            ah[e] = x1;
+            bh[e] = y1;
+            al[e] = (__bf16)(x - (float)x1);
+            bl[e] = (__bf16)(y - (float)y1);
           }
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[q], 0, 0, 0);
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[q], 0, 0, 0);
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[q], 0, 0, 0);
         }
+      }
+    }
+    if (kTwoLevel) {  // panel sum -> running total
 #pragma unroll
-        for (int a = 0; a < TW; ++a)
-#pragma unroll
-          for (int c = 0; c < TW; ++c) {
-            if (!live[a][c]) continue;
-            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
-          }
+      for (int q = 0; q < NT; ++q) {
+        tot[q] += acc[q];
+        acc[q] = mf_v16{};
       }
     }
     __syncthreads();
@@ -426,17 +445,17 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   }
   double* Gp = G + (long long)blockIdx.x * K * K;
 #pragma unroll
-  for (int a = 0; a < TW; ++a)
+  for (int q = 0; q < NT; ++q) {
+    if (!live[q]) continue;
+    const mf_v16 v = kTwoLevel ? tot[q] : acc[q];
+    const int col = tc[q] + (lane & 31);
 #pragma unroll
-    for (int c = 0; c < TW; ++c) {
-      const int col = c0 + 32 * c + (lane & 31);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = r0 + 32 * a + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        if (row < N && col < N && row <= col)  // upper triangle: all its readers use
-          Gp[row * N + col] = (double)acc[a][c][r] + (!dm && row == col ? diag : 0.0);
-      }
+    for (int r = 0; r < 16; ++r) {
+      const int row = tr[q] + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+      if (row < N && col < N && row <= col)  // upper triangle: all its readers use
+        Gp[row * N + col] = (double)v[r] + (!dm && row == col ? diag : 0.0);
     }
+  }
 }
 
 template <int PREC, typename T>
@@ -461,6 +480,8 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
   if (prec == LETKF_PREC_FP32)
     launch_gram_mixed_p<LETKF_PREC_FP32, T>(kp.K, diag, src, na, G, bvec, dual, s);
+  else if (prec == LETKF_PREC_FP32_PANEL)
+    launch_gram_mixed_p<LETKF_PREC_FP32_PANEL, T>(kp.K, diag, src, na, G, bvec, dual, s);
   else
     launch_gram_mixed_p<LETKF_PREC_BF16X3, T>(kp.K, diag, src, na, G, bvec, dual, s);
   return hipGetLastError();

@@ -81,7 +81,8 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
                             hipStream_t s);
 
 // The same products with the Gram on MFMA in FP32 (prec LETKF_PREC_FP32)
-// or from BF16 hi/lo splits (LETKF_PREC_BF16X3), FP32 accumulation, b in
+// or from BF16 hi/lo splits (LETKF_PREC_BF16X3), FP32 accumulation (per
+// panel into FP32 totals for LETKF_PREC_FP32_PANEL), b in
 // FP64; K <= 128.
 template <typename T>
 hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb,
