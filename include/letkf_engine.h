@@ -42,10 +42,13 @@ typedef enum letkf_precision {
   LETKF_PREC_FP32 = 1,      /* Gram Y^T R^-1 Y on FP32 MFMA, FP64 eig    */
   LETKF_PREC_BF16X3 = 2,    /* Gram from hi/lo BF16 splits on MFMA with  */
                             /* FP32 accumulation, FP64 eig               */
-  LETKF_PREC_FP32_PANEL = 3 /* as FP32, the accumulators summing one     */
+  LETKF_PREC_FP32_PANEL = 3,/* as FP32, the accumulators summing one     */
                             /* panel of 32 obs at a time into FP32       */
                             /* running totals: 2-6x smaller Gram         */
                             /* rounding error, fewer waves per SIMD      */
+  LETKF_PREC_FP32X3 = 4     /* Gram from FP32 hi/lo splits of the FP64   */
+                            /* operands, hi*hi + hi*lo + lo*hi on FP32   */
+                            /* MFMA with FP32 accumulation, FP64 eig     */
 } letkf_precision_t;
 
 /* How the per-point ensemble-space problem is solved. */

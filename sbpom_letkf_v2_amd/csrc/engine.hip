@@ -46,7 +46,7 @@ static int check_params(const letkf_params_t* p) {
     return set_err(LETKF_ERR_ARG, "nv3d+nv2d must be in [1,8] (got %d+%d)", p->nv3d, p->nv2d);
   if (!(p->cov_infl_mul > 0.0)) return set_err(LETKF_ERR_ARG, "cov_infl_mul must be > 0");
   if (p->precision != LETKF_PREC_FP64 && p->precision != LETKF_PREC_FP32 && p->precision != LETKF_PREC_BF16X3 &&
-      p->precision != LETKF_PREC_FP32_PANEL)
+      p->precision != LETKF_PREC_FP32_PANEL && p->precision != LETKF_PREC_FP32X3)
     return set_err(LETKF_ERR_ARG, "unknown precision %d", p->precision);
   if (p->form != LETKF_FORM_PLAIN && p->form != LETKF_FORM_BATCHED)
     return set_err(LETKF_ERR_ARG, "unknown form %d", p->form);

@@ -264,6 +264,10 @@ template hipError_t launch_gram_fp64<double>(const KParams&, const Tables<double
 // panel sum or to the total rather than to a running sum over hundreds of
 // terms (2-6x smaller Gram errors at 400-800 local obs), at the price of
 // 16 more registers per tile (occupancy 8 -> 6 at OUT 64, 4 -> 2 at 128).
+// prec FP32X3 removes the operands' FP32 rounding instead: each operand is
+// staged as hi = fp32(x) and lo = fp32(x - hi) (panels of 16 rows, so that
+// the four panels take the LDS of two) and every tile sums hi*hi + hi*lo +
+// lo*hi on v_mfma_f32_32x32x2f32.
 // Lane layouts (checked on gfx950):
 //   32x32x2 f32 : a = A[i][k], i = lane%32, k = lane/32;  b = B[k][j], j = lane%32
 //   32x32x16 bf16: a = A[i][8*(lane/32)+q], b = B[8*(lane/32)+q][j], q = 0..7
@@ -285,15 +289,21 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
                                                    double* __restrict__ bvec, int dual) {
   constexpr int NT = OUT == 128 ? 3 : 1;          // tiles per wavefront
   constexpr bool kTwoLevel = PREC == LETKF_PREC_FP32_PANEL;
+  constexpr bool kSplit = PREC == LETKF_PREC_FP32X3;  // operands as FP32 hi + lo
+  // the split keeps a lo panel per operand in the LDS of half-height panels
+  constexpr int kPanel = kSplit ? kMfPanel / 2 : kMfPanel;
+  constexpr int kLoRows = kSplit ? kPanel : 1;
   constexpr int LP = OUT + 1;    // odd pitch: row-wise and column-wise panel writes hit distinct banks
   constexpr int RS = 256 / OUT;  // primal: threads sharing one output column
-  constexpr int kStage = kMfPanel * OUT / 256;  // panel elements staged per thread (8 or 16)
-  constexpr int kBatch = OUT == 128 ? 8 : 1;    // loads issued together (OUT 64 keeps occupancy 8)
+  constexpr int kStage = kPanel * OUT / 256;  // panel elements staged per thread (4 .. 16)
+  constexpr int kBatch = OUT == 128 ? (kSplit ? 4 : 8) : 1;  // loads issued together (occupancy: 8 at OUT 64)
   // panel: rows = summation index (obs or member), cols = output index
-  __shared__ float As[kMfPanel][LP];
-  __shared__ float Bs[kMfPanel][LP];
+  __shared__ float As[kPanel][LP];
+  __shared__ float Bs[kPanel][LP];
+  __shared__ float Al[kLoRows][LP];  // FP32X3: x - (double)hi rounded to FP32
+  __shared__ float Bl[kLoRows][LP];
   __shared__ double sws[OUT];  // dual: sqrt weight per obs; primal: weight per panel row
-  __shared__ double dps[kMfPanel];
+  __shared__ double dps[kPanel];
   __shared__ int ios[OUT];
   __shared__ double bred[256];
   const int n = src.begin(blockIdx.x);
@@ -332,8 +342,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   // primal: b = Y^T diag(rloc/rdiag) dep in FP64, thread t summing column
   // t % OUT over the panel rows it stages
   double bpart = 0.0;
-  for (int s0 = 0; s0 < S; s0 += kMfPanel) {
-    const int ps = min(kMfPanel, S - s0);
+  for (int s0 = 0; s0 < S; s0 += kPanel) {
+    const int ps = min(kPanel, S - s0);
     if (dm) {
       // B^T panel: lanes walk the members of one obs (contiguous in yT);
       // a batch of kBatch loads is issued before its first store
@@ -342,16 +352,22 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
         double yv[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-          const int q = t + 256 * (u0 + u), row = q & (kMfPanel - 1), col = q / kMfPanel;
+          const int q = t + 256 * (u0 + u), row = q & (kPanel - 1), col = q / kPanel;
           yv[u] = 0.0;
           if (row < ps && col < N) src.row_of(ios[col], K, s0 + row, yv[u]);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-          const int q = t + 256 * (u0 + u), row = q & (kMfPanel - 1), col = q / kMfPanel;
-          const float v = col < N ? (float)(sws[col] * yv[u]) : 0.f;
+          const int q = t + 256 * (u0 + u), row = q & (kPanel - 1), col = q / kPanel;
+          const double x = col < N ? sws[col] * yv[u] : 0.0;
+          const float v = (float)x;
           As[row][col] = v;
           Bs[row][col] = v;
+          if (kSplit) {
+            const float lo = (float)(x - (double)v);
+            Al[row][col] = lo;
+            Bl[row][col] = lo;
+          }
         }
       }
     } else {
@@ -377,20 +393,29 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
           const int q = t + 256 * (u0 + u), row = q / OUT, col = q % OUT;
-          float av = 0.f, bv = 0.f;
+          float av = 0.f, bv = 0.f, al = 0.f, bl = 0.f;
           if (row < ps && col < N) {
             const double wy = yv[u] * sws[row];
             bpart = fma(wy, dps[row], bpart);
             av = (float)wy;
             bv = (float)yv[u];
+            if (kSplit) {
+              al = (float)(wy - (double)av);
+              bl = (float)(yv[u] - (double)bv);
+            }
           }
           As[row][col] = av;
           Bs[row][col] = bv;
+          if (kSplit) {
+            Al[row][col] = al;
+            Bl[row][col] = bl;
+          }
         }
       }
     }
     __syncthreads();
     if (PREC != LETKF_PREC_BF16X3) {
+#pragma unroll 2
       for (int k = 0; k < ps; k += 2) {
         const int kk = k + (lane >> 5);
 #pragma unroll
@@ -399,6 +424,12 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
           const float av = kk < ps ? As[kk][tr[q] + (lane & 31)] : 0.f;
           const float bv = kk < ps ? Bs[kk][tc[q] + (lane & 31)] : 0.f;
           acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[q], 0, 0, 0);
+          if (kSplit) {  // hi*lo + lo*hi: the operands to ~2^-48
+            const float al = kk < ps ? Al[kk][tr[q] + (lane & 31)] : 0.f;
+            const float bl = kk < ps ? Bl[kk][tc[q] + (lane & 31)] : 0.f;
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bl, acc[q], 0, 0, 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(al, bv, acc[q], 0, 0, 0);
+          }
         }
       }
     } else {
@@ -482,6 +513,8 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
     launch_gram_mixed_p<LETKF_PREC_FP32, T>(kp.K, diag, src, na, G, bvec, dual, s);
   else if (prec == LETKF_PREC_FP32_PANEL)
     launch_gram_mixed_p<LETKF_PREC_FP32_PANEL, T>(kp.K, diag, src, na, G, bvec, dual, s);
+  else if (prec == LETKF_PREC_FP32X3)
+    launch_gram_mixed_p<LETKF_PREC_FP32X3, T>(kp.K, diag, src, na, G, bvec, dual, s);
   else
     launch_gram_mixed_p<LETKF_PREC_BF16X3, T>(kp.K, diag, src, na, G, bvec, dual, s);
   return hipGetLastError();
