@@ -25,7 +25,7 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 
 METRIC = "analysed grid pts/s at K=64: plain FP64 vs batched mixed-prec; max err vs FP64"
 PEAK_FP64_TFLOPS = 78.6
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp32p": 157.3, "fp32x3": 157.3, "bf16x3": 2500.0}   # dense, per operand type of the Gram
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp32p": 157.3, "fp32x3": 157.3, "fp32d": 157.3, "bf16x3": 2500.0}   # dense, per operand type of the Gram
 PEAK_HBM_GBS = 8000.0
 EIG_KINDS = ["k_eigr_tridiag", "k_eigr_ql", "k_eigr_qvec", "k_eigr_vrot", "k_eigr_vout"]
 
@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp32p", "fp32x3", "bf16x3"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp32p", "fp32x3", "fp32d", "bf16x3"])
     ap.add_argument("--form", default="batched", choices=["plain", "batched"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle baseline sample")

@@ -46,9 +46,11 @@ typedef enum letkf_precision {
                             /* panel of 32 obs at a time into FP32       */
                             /* running totals: 2-6x smaller Gram         */
                             /* rounding error, fewer waves per SIMD      */
-  LETKF_PREC_FP32X3 = 4     /* Gram from FP32 hi/lo splits of the FP64   */
+  LETKF_PREC_FP32X3 = 4,    /* Gram from FP32 hi/lo splits of the FP64   */
                             /* operands, hi*hi + hi*lo + lo*hi on FP32   */
                             /* MFMA with FP32 accumulation, FP64 eig     */
+  LETKF_PREC_FP32_DUAL64 = 5 /* FP32 MFMA Gram for the K x K form, FP64  */
+                            /* Gram for the n x n dual form (batched)    */
 } letkf_precision_t;
 
 /* How the per-point ensemble-space problem is solved. */

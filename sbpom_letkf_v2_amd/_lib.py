@@ -16,7 +16,7 @@ STATUS_NAMES = {0: "LETKF_OK", -1: "LETKF_ERR_ARG", -2: "LETKF_ERR_HIP",
                 -3: "LETKF_ERR_EIG", -4: "LETKF_ERR_CAPACITY",
                 -5: "LETKF_ERR_NONFINITE"}
 
-PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP32_PANEL, PREC_FP32X3 = 0, 1, 2, 3, 4
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP32_PANEL, PREC_FP32X3, PREC_FP32_DUAL64 = 0, 1, 2, 3, 4, 5
 FORM_PLAIN, FORM_BATCHED = 0, 1
 
 # every symbol include/letkf_engine.h declares

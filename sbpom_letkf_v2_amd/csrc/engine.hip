@@ -46,7 +46,8 @@ static int check_params(const letkf_params_t* p) {
     return set_err(LETKF_ERR_ARG, "nv3d+nv2d must be in [1,8] (got %d+%d)", p->nv3d, p->nv2d);
   if (!(p->cov_infl_mul > 0.0)) return set_err(LETKF_ERR_ARG, "cov_infl_mul must be > 0");
   if (p->precision != LETKF_PREC_FP64 && p->precision != LETKF_PREC_FP32 && p->precision != LETKF_PREC_BF16X3 &&
-      p->precision != LETKF_PREC_FP32_PANEL && p->precision != LETKF_PREC_FP32X3)
+      p->precision != LETKF_PREC_FP32_PANEL && p->precision != LETKF_PREC_FP32X3 &&
+      p->precision != LETKF_PREC_FP32_DUAL64)
     return set_err(LETKF_ERR_ARG, "unknown precision %d", p->precision);
   if (p->form != LETKF_FORM_PLAIN && p->form != LETKF_FORM_BATCHED)
     return set_err(LETKF_ERR_ARG, "unknown form %d", p->form);
@@ -270,6 +271,25 @@ __global__ void __launch_bounds__(256) k_bucket_scatter(long long P, const int32
     }
     __syncthreads();
   }
+}
+
+// The Gram of active slots [a0, a0+na) in the handle's precision: FP64, the
+// MFMA kernel (K <= 128), or the MFMA kernel for the K x K slots and the FP64
+// kernel for the dual n x n ones.
+template <typename T>
+static hipError_t launch_gram_any(int prec, const KParams& kp, const Tables<T>& tb, const double* yT,
+                                  const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
+                                  const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
+                                  hipStream_t s) {
+  if (prec == LETKF_PREC_FP64 || kp.K > 128)
+    return launch_gram_fp64<T>(kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec, dual, s);
+  if (prec != LETKF_PREC_FP32_DUAL64 || !dual)
+    return launch_gram_mixed<T>(prec == LETKF_PREC_FP32_DUAL64 ? LETKF_PREC_FP32 : prec, kp, tb, yT, act_p, loff,
+                                lbase, lobs, lrloc, a0, na, G, bvec, dual, s);
+  hipError_t e = launch_gram_mixed<T>(LETKF_PREC_FP32, kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec,
+                                      2, s);
+  if (e != hipSuccess) return e;
+  return launch_gram_fp64<T>(kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec, 2, s);
 }
 
 template <typename T>
@@ -529,14 +549,9 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
     }
     HIP_TRY(mark(e0 + 1));
     if (na > 0) {
-      if (p.precision != LETKF_PREC_FP64 && K <= 128)
-        HIP_TRY(launch_gram_mixed<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
-                                     h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0,
-                                     na, h->G.as<double>(), h->bvec.as<double>(), dual, s));
-      else
-        HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
-                                    lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
-                                    h->G.as<double>(), h->bvec.as<double>(), dual, s));
+      HIP_TRY(launch_gram_any<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
+                                 h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
+                                 h->G.as<double>(), h->bvec.as<double>(), dual, s));
     }
     HIP_TRY(mark(e0 + 2));
     if (na > 0) {
@@ -796,14 +811,9 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
       HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
       HIP_TRY(launch_search_fill<T>(kp, tb, pl.bpts[0], pl.bpts[1], h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
                                     h->lrloc.as<double>(), s));
-      if (p.precision != LETKF_PREC_FP64)
-        HIP_TRY(launch_gram_mixed<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
-                                     h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0,
-                                     na, h->G.as<double>(), h->bvec.as<double>(), dual, s));
-      else
-        HIP_TRY(launch_gram_fp64<T>(kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
-                                    lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
-                                    h->G.as<double>(), h->bvec.as<double>(), dual, s));
+      HIP_TRY(launch_gram_any<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
+                                 h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
+                                 h->G.as<double>(), h->bvec.as<double>(), dual, s));
       hipEvent_t e0, e1;
       HIP_TRY(hipEventCreate(&e0));
       HIP_TRY(hipEventCreate(&e1));

@@ -87,6 +87,8 @@ struct CsrSource {
 // KT: padded ensemble tile (64 or 128); each of the 16x16 threads owns a
 // TT x TT block of G with TT = KT/16.  diag = (K-1)/rho in r_size.
 //
+// dual == 2: only the points with n < K (the others are left to the MFMA
+// Gram, which then skips these).
 // dual != 0 (batched form): a point with n < K local obs instead gets the
 // n x n matrix M = B B^T, B = diag(sqrt(rloc/rdiag)) Y, summed over the K
 // members, and bvec = diag(sqrt(rloc/rdiag)) dep; both are stored with
@@ -103,6 +105,7 @@ __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, 
   __shared__ double sws[KT];
   __shared__ int ios[KT];
   const int n = src.begin(blockIdx.x);
+  if (dual == 2 && n >= K) return;  // dual slots only (the MFMA kernel takes the others)
   const int t = threadIdx.x;
   const int tr = t >> 4, tc = t & 15;
   double acc[TT][TT];
@@ -310,6 +313,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: tile tables in scalar registers
   const bool dm = dual && n < K;
+  if (dual == 2 && dm) return;  // K x K slots only (the FP64 kernel takes the dual ones)
   const int N = dm ? n : K;  // output dimension
   const int S = dm ? K : n;  // summation length
   // this wavefront's tiles: row / column offsets and liveness (wave-uniform)

@@ -240,7 +240,8 @@ def test_rotation_log_overflow_fallback(monkeypatch):
                                                    ("fp32", "plain", 48, 900), ("fp32", "batched", 100, 1800),
                                                    ("bf16x3", "batched", 128, 2000), ("fp32", "plain", 100, 1800),
                                                    ("fp32p", "batched", 48, 900), ("fp32p", "plain", 100, 1800),
-                                                   ("fp32x3", "batched", 48, 900), ("fp32x3", "batched", 100, 1800)])
+                                                   ("fp32x3", "batched", 48, 900), ("fp32x3", "batched", 100, 1800),
+                                                   ("fp32d", "batched", 48, 900), ("fp32d", "batched", 100, 1800)])
 def test_mixed_precision_gram_close_to_fp64(precision, form, K, nobs):
     """Gram on MFMA (FP32, FP32 summed per panel, FP32 or BF16 hi/lo splits) with FP64 eigensolve: the
     analysis must stay within 1e-4 of the FP64 oracle relative to the field
