@@ -240,9 +240,12 @@ __device__ bool eig_wave(int n, double* Z, int ld, double* d, double* e) {
 // Descending order (ties keep the reverse of DSYEV's ascending order),
 // negative eigenvalues set to 0 with their vectors (common.f90:1030-1043),
 // status bits for no convergence / all negative / non-finite.  lb gets the
-// n eigenvalues, Vb the eigenvectors with row pitch n.
+// n eigenvalues, Vb the eigenvectors with row pitch n.  keep_neg (the dual
+// n x n matrix M = B B^T, semi-definite): negative eigenvalues are rounding
+// of M and are kept with their vectors (the update uses alpha + s), and an
+// all-negative spectrum is no error.
 __device__ void finish_eig(int n, const double* Z, int ld, const double* d, double* lb, double* Vb, bool ok,
-                           int* status) {
+                           int* status, bool keep_neg) {
   const int lane = threadIdx.x & 63;
   double dmax = -INFINITY;
   bool finite = true;
@@ -254,14 +257,14 @@ __device__ void finish_eig(int n, const double* Z, int ld, const double* d, doub
     const double dj = d[j];
     int rank = 0;
     for (int c = 0; c < n; ++c) rank += (d[c] > dj) || (d[c] == dj && c > j);
-    const bool neg = dj < 0.0;
+    const bool neg = !keep_neg && dj < 0.0;
     lb[rank] = neg ? 0.0 : dj;
     for (int r = 0; r < n; ++r) Vb[r * n + rank] = neg ? 0.0 : Z[r * ld + j];
   }
   if (lane == 0) {
     int st = 0;
     if (!ok) st |= ST_EIG_NOCONV;
-    if (dmax < 0.0) st |= ST_EIG_NEG;
+    if (!keep_neg && dmax < 0.0) st |= ST_EIG_NEG;
     if (!finite) st |= ST_NONFINITE;
     if (st) atomicOr(status, st);
   }
@@ -303,7 +306,8 @@ __global__ void __launch_bounds__(64) k_eig_fp64(int K, const double* __restrict
   } else {
     ok = eig_wave(n, Z, ld, d, e);
   }
-  finish_eig(n, Z, ld, d, lam + (long long)blockIdx.x * K, V + (long long)blockIdx.x * K * K, ok, status);
+  finish_eig(n, Z, ld, d, lam + (long long)blockIdx.x * K, V + (long long)blockIdx.x * K * K, ok, status,
+             dual && n < K);
 }
 
 hipError_t launch_eig_fp64(int K, int na, const double* G, double* lam, double* V, int* status, int dual,
@@ -555,7 +559,7 @@ __global__ void __launch_bounds__(64) k_eig_apply(EigSlots es, const double* __r
     }
     __syncthreads();
   }
-  finish_eig(n, Z, ld, d, lam + (long long)b * K, V + (long long)b * K * K, ok, status);
+  finish_eig(n, Z, ld, d, lam + (long long)b * K, V + (long long)b * K * K, ok, status, es.dual && n < es.K);
 }
 
 // Rotations logged per matrix (capacity); a typical implicit-QL solve needs

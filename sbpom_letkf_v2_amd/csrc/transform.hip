@@ -256,14 +256,16 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
       const double al = dr.alpha, sal = sqrt(al);
       if (g0) {
         const double sc = sb[lane], ap_ = al + sc, sap = sqrt(ap_);
-        const double gc = -1.0 / (sal * sap * (sal + sap));
-        q_s[lane] = qd0 / ap_;
+        const bool ok = ap_ > 0.0;  // s of M = B B^T above -alpha (rounding keeps it there)
+        const double gc = ok ? -1.0 / (sal * sap * (sal + sap)) : 0.0;
+        q_s[lane] = ok ? qd0 / ap_ : 0.0;
         _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) z_s[v * K + lane] = gc * u0[v];
       }
       if (g1) {
         const double sc = sb[lane + 64], ap_ = al + sc, sap = sqrt(ap_);
-        const double gc = -1.0 / (sal * sap * (sal + sap));
-        q_s[lane + 64] = qd1 / ap_;
+        const bool ok = ap_ > 0.0;
+        const double gc = ok ? -1.0 / (sal * sap * (sal + sap)) : 0.0;
+        q_s[lane + 64] = ok ? qd1 / ap_ : 0.0;
         _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) z_s[v * K + lane + 64] = gc * u1[v];
       }
       wave_sync();
