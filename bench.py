@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--setobs", action="store_true",
                     help="time set_obs after H(x) (default workload c1): statistics, QC, compaction, cell "
                          "sort, AOEI over every obs, with a bit-exact oracle check and CPU baseline")
+    ap.add_argument("--output", action="store_true",
+                    help="time the device half of the output step (default workload c1): grid mean/spread "
+                         "of analysis and forecast and the REAL(4) file images of every field, with a "
+                         "bit-exact oracle check and CPU baseline on two levels")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise kernel that reads and writes 1 GiB (three launches), the "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -233,6 +237,117 @@ def run_setobs(args):
                                   "every member"},
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": gbs / PEAK_HBM_GBS, "traffic": None},
+           "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+def run_output(args):
+    """The device half of the output step after the analysis (write_ens_mpi,
+    common_io.f90:863-1074) on a workload (default c1): one step = grid mean
+    and spread of the analysis and of the forecast (ensemble_mesp,
+    common.f90:505-596) + the REAL(4) file image of every written field
+    (nmem analysis members, the two means and the two spreads: gather to the
+    global grid, quasi-global wrap, file vertical order).  Roofline: HBM,
+    algorithmic bytes = each ensemble read once and the mean/spread written
+    (2 x (nmem + 2) x 8 B per grid value) plus 8 B read and 4 B written per
+    value of each packed field."""
+    import numpy as np
+    import torch
+    import pyoracle as po
+    from sbpom_letkf_v2_amd import engine, synth
+    wl = args.workload if args.workload != "c2" else "c1"
+    log(f"generating workload {wl} on the GPU")
+    pr = synth.config(wl, device="cuda", real_kind=8)
+    meta = pr.meta
+    nlon, nlat, nlev, K = meta["nlon"], meta["nlat"], meta["nlev"], meta["nmem"]
+    nv3d, nv2d, nij = meta["nv3d"], meta["nv2d"], nlon * nlat
+    f3, f2 = pr.fcst3d, pr.fcst2d
+    if nv2d == 0:
+        f2 = torch.zeros((1, K, nij), dtype=torch.float64, device="cuda")
+        nv2d = 1
+    # an analysis ensemble of the same shape (values do not change the cost)
+    a3 = (f3 * 0.75).contiguous()
+    a2 = (f2 * 0.75).contiguous()
+    g3 = torch.empty((K + 4, nv3d, nlev, nlat, nlon), dtype=torch.float32, device="cuda")
+    g2 = torch.empty((K + 4, nv2d, nlat, nlon), dtype=torch.float32, device="cuda")
+    nval = nij * (nlev * nv3d + nv2d)
+    log(f"  {nlon}x{nlat}x{nlev} nmem {K}: ensembles {2 * f3.numel() * 8 / 1e9:.1f} GB, file images "
+        f"{(g3.numel() + g2.numel()) * 4 / 1e9:.1f} GB")
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def step(t):
+        t[0].record()
+        am3, as3, am2, as2 = engine.ensemble_mesp_device(a3, a2)
+        fm3, fs3, fm2, fs2 = engine.ensemble_mesp_device(f3, f2)
+        t[1].record()
+        engine.grid_pack_device(nlon, nlat, a3, a2, wrap=True, file_order=True, g3=g3[:K], g2=g2[:K])
+        for j, (x3, x2) in enumerate(((am3, am2), (as3, as2), (fm3, fm2), (fs3, fs2))):
+            engine.grid_pack_device(nlon, nlat, x3, x2, wrap=True, file_order=True, g3=g3[K + j:K + j + 1],
+                                    g2=g2[K + j:K + j + 1])
+        t[2].record()
+        return am3, as3, am2, as2
+
+    for _ in range(args.warmup):
+        step([ev(), ev(), ev()])
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(args.steps):
+        t = [ev(), ev(), ev()]
+        res = step(t)
+        ts.append(t)
+    torch.cuda.synchronize()
+    ms_mesp = sum(t[0].elapsed_time(t[1]) for t in ts) / args.steps
+    ms_pack = sum(t[1].elapsed_time(t[2]) for t in ts) / args.steps
+    ms = ms_mesp + ms_pack
+    b_mesp = 2.0 * nval * (K + 2) * 8
+    b_pack = (K + 4.0) * nval * 12
+    gbs = lambda b, t: b / (t * 1e-3) / 1e9
+    log(f"  mesp x2 {ms_mesp:.2f} ms ({gbs(b_mesp, ms_mesp):.0f} GB/s algorithmic), pack of {K + 4} fields "
+        f"{ms_pack:.2f} ms ({gbs(b_pack, ms_pack):.0f} GB/s), {nij * nlev / (ms * 1e-3):.3e} grid points/s")
+    cpu = None
+    if not args.no_cpu:
+        # oracle on two levels (every member, every variable): mesp of both
+        # ensembles and the file images of every field, bit for bit against
+        # the GPU's output levels nlev-2, nlev-1 (the file order flips them)
+        L = 2
+        host3 = lambda t: np.asfortranarray(t[:, :, :L].cpu().numpy().transpose(3, 2, 1, 0))  # (nij,L,K,nv3d)
+        host2 = lambda t: np.asfortranarray(t.cpu().numpy().transpose(2, 1, 0))
+        ha3, hf3, ha2, hf2 = host3(a3), host3(f3), host2(a2), host2(f2)
+        t0 = time.perf_counter()
+        wa = po.ensemble_mesp(ha3, ha2)
+        wf = po.ensemble_mesp(hf3, hf2)
+        p3, p2 = po.grid_pack(nlon, nlat, ha3, ha2, wrap=True, file_order=True)
+        x3 = np.asfortranarray(np.stack([wa[0], wa[1], wf[0], wf[1]], axis=2))
+        x2 = np.asfortranarray(np.stack([wa[2], wa[3], wf[2], wf[3]], axis=1))
+        q3, q2 = po.grid_pack(nlon, nlat, x3, x2, wrap=True, file_order=True)
+        t_cpu = time.perf_counter() - t0
+        am3, as3, am2, as2 = res
+        ok = (np.array_equal(am3[:, :L].cpu().numpy(), wa[0].transpose(2, 1, 0))
+              and np.array_equal(as3[:, :L].cpu().numpy(), wa[1].transpose(2, 1, 0))
+              and np.array_equal(am2.cpu().numpy(), wa[2].T) and np.array_equal(as2.cpu().numpy(), wa[3].T)
+              and np.array_equal(g3[:K, :, nlev - L:].cpu().numpy(), p3.transpose(4, 3, 2, 1, 0))
+              and np.array_equal(g3[K:, :, nlev - L:].cpu().numpy(), q3.transpose(4, 3, 2, 1, 0))
+              and np.array_equal(g2[:K].cpu().numpy(), p2.transpose(3, 2, 1, 0))
+              and np.array_equal(g2[K:].cpu().numpy(), q2.transpose(3, 2, 1, 0)))
+        # the 2-D variables are done in full on both sides; the CPU rate is
+        # per 3-D grid point of the two levels
+        cpu = {"value": nij * L / t_cpu, "unit": "grid points/s", "cores": 1, "kind": "port",
+               "sample": f"{L} of {nlev} levels, all {K} members and variables, FP64 C oracle (one thread), "
+                         f"{t_cpu:.2f} s"}
+        log(f"  oracle on {L} levels: bit-exact {ok}; cpu {cpu['value']:.3e} grid points/s")
+        if not ok:
+            raise SystemExit("output step differs from the oracle")
+    out = {"metric": "output step (grid mean/spread + file images) grid points per second",
+           "value": nij * nlev / (ms * 1e-3), "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{wl}: {nlon}x{nlat}x{nlev} K={K} nv3d={nv3d} nv2d={nv2d}, mesp of analysis "
+                                  f"and forecast + REAL(4) file images of {K + 4} fields"},
+           "stage_ms": {"mesp_x2": ms_mesp, "pack": ms_pack},
+           "roofline": {"bound": "hbm", "achieved": gbs(b_mesp + b_pack, ms), "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": gbs(b_mesp + b_pack, ms) / PEAK_HBM_GBS, "traffic": None},
+           "kernel_roofline": {"k_mesp": {"ms": ms_mesp, "achieved": gbs(b_mesp, ms_mesp), "unit": "GB/s"},
+                               "k_grid_pack": {"ms": ms_pack, "achieved": gbs(b_pack, ms_pack), "unit": "GB/s"}},
            "cpu_baseline": cpu}
     print(json.dumps(out))
 
@@ -552,6 +667,8 @@ def main():
         return
     if args.setobs:
         return run_setobs(args)
+    if args.output:
+        return run_output(args)
     if args.xtoy:
         run_xtoy(args)
         return

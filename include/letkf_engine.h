@@ -354,6 +354,43 @@ int letkf_set_obs_device(letkf_handle_t* h, const letkf_setobs_cfg_t* cfg,
                          const void* hx, const int32_t* qc,
                          letkf_obs_set_t* out);
 
+/* ensemble_mesp (common.f90:505-596) on DEVICE arrays, as write_ens_mpi
+ * calls it for the analysis and the forecast (common_io.f90:922-924):
+ * v3d(nij1,nlev,nmem,nv3d), v2d(nij1,nmem,nv2d) in, v3dm/v3ds
+ * (nij1,nlev,nv3d) and v2dm/v2ds (nij1,nv2d) out, Fortran order, element
+ * type of real_kind (4 or 8).  Mean = member sum in member order / nmem;
+ * spread = sqrt(sum of squared deviations from that mean / (nmem - 1)),
+ * undef when nmem = 1; bit-identical to the reference's loops.  Runs on
+ * `stream` (NULL: the default stream) and returns when done. */
+int letkf_ensemble_mesp_device(int32_t real_kind, int32_t nij1, int32_t nlev,
+                               int32_t nmem, int32_t nv3d, int32_t nv2d,
+                               double undef, const void* v3d, const void* v2d,
+                               void* v3dm, void* v3ds, void* v2dm, void* v2ds,
+                               void* stream);
+
+/* The REAL(4) global grids of nfield fields as the output step forms them:
+ * gather_grd_mpi (common_mpi.f90:133-206) with one rank, where local value
+ * i is grid point (ilon, ilat), i - 1 = (ilon - 1) + nlon (ilat - 1); then,
+ * when wrap, the quasi-global copy of columns 1:2 into nlon-1:nlon
+ * (lqglobal, common_io.f90:960-963; needs nlon >= 4); when file_order, the
+ * vertical order of the written file (level k to nlev - k + 1,
+ * write_state_vector_all, common_io.f90:655-661).  Field f, 3-D variable v,
+ * level k, local value i is read at v3d[s3[0] f + s3[1] v + s3[2] k + i],
+ * 2-D variable v at v2d[s2[0] f + s2[1] v + i] (strides in elements; the
+ * analysis arrays anal3d(nij1,nlev,nmem,nv3d), the members being the
+ * fields, have s3 = {nij1 nlev, nij1 nlev nmem, nij1}, s2 = {nij1, nij1
+ * nmem}).  g3 receives (nlon,nlat,nlev,nv3d) and g2 (nlon,nlat,nv2d) per
+ * field, packed.  Without file_order the grids are the v3dg/v2dg that
+ * Trans_XtoY reads for H(x^a) (common_io.f90:966-989).  Runs on `stream`
+ * and returns when done.  The reference has no single counterpart: it
+ * gathers one field at a time over MPI. */
+int letkf_grid_pack_device(int32_t real_kind, int32_t nlon, int32_t nlat,
+                           int32_t nlev, int32_t nv3d, int32_t nv2d,
+                           int32_t nfield, int32_t wrap, int32_t file_order,
+                           const void* v3d, const int64_t* s3, const void* v2d,
+                           const int64_t* s2, float* g3, float* g2,
+                           void* stream);
+
 /* Adjoint of letkf_da_main_device (no counterpart in the reference; for
  * gradient-based uses of the analysis: observation impact, tuning of the
  * inflation and of the observation errors, torch.autograd).  DEVICE arrays;

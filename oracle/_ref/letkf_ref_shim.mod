@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:69b4082244a92dd2
+﻿!mod$ v1 sum:f584c75948b673c5
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ 2dcd5c82c7cba8f9 n common
 !need$ 7d9124a989da7208 n common_letkf
@@ -30,10 +30,12 @@ use common_setting,only:iv3d_v
 use common_setting,only:iv3d_t
 use common_setting,only:iv3d_s
 use common_setting,only:iv2d_z
+use common_setting,only:nij1
 use common,only:distance_2p
 use common,only:mtx_eigen
 use common,only:akima_spline
 use common,only:trans_xtoy
+use common,only:ensemble_mesp
 use common_letkf,only:letkf_core
 use,intrinsic::iso_c_binding,only:c_associated
 use,intrinsic::iso_c_binding,only:c_funloc
@@ -169,5 +171,14 @@ end
 subroutine ref_constants(pi_out,re_out) bind(c,name="ref_constants")
 real(4),intent(out)::pi_out
 real(4),intent(out)::re_out
+end
+subroutine ref_ensemble_mesp(n,v3d,v2d,v3dm,v3ds,v2dm,v2ds) bind(c,name="ref_ensemble_mesp")
+integer(4),value::n
+real(4),intent(in)::v3d(1_8:int(n,kind=8),1_8:75_8,1_8:128_8,1_8:4_8)
+real(4),intent(in)::v2d(1_8:int(n,kind=8),1_8:128_8,1_8:3_8)
+real(4),intent(out)::v3dm(1_8:int(n,kind=8),1_8:75_8,1_8:4_8)
+real(4),intent(out)::v3ds(1_8:int(n,kind=8),1_8:75_8,1_8:4_8)
+real(4),intent(out)::v2dm(1_8:int(n,kind=8),1_8:3_8)
+real(4),intent(out)::v2ds(1_8:int(n,kind=8),1_8:3_8)
 end
 end

@@ -121,7 +121,16 @@ typedef struct oracle_tables_f32 { ORACLE_TABLES(float) } oracle_tables_f32_t;
                           int32_t* o_idx, int32_t* o_idy, REAL* o_lon,         \
                           REAL* o_lat, REAL* o_lev, REAL* o_dat, REAL* o_err,  \
                           REAL* o_dep, REAL* o_mean, REAL* o_sprd,             \
-                          REAL* o_hdxf, int32_t* nobsgrd, int32_t* ndel);
+                          REAL* o_hdxf, int32_t* nobsgrd, int32_t* ndel);   \
+  void oracle_ensemble_mesp##SUF(int nij1, int nlev, int nmem, int nv3d,        \
+                                 int nv2d, REAL undef, const REAL* v3d,         \
+                                 const REAL* v2d, REAL* v3dm, REAL* v3ds,       \
+                                 REAL* v2dm, REAL* v2ds);                       \
+  void oracle_grid_pack##SUF(int nlon, int nlat, int nlev, int nv3d, int nv2d,  \
+                             int nfield, int wrap, int file_order,              \
+                             const REAL* v3d, int64_t f3, int64_t s3v,          \
+                             int64_t s3k, const REAL* v2d, int64_t f2,          \
+                             int64_t s2v, float* g3, float* g2);
 
 ORACLE_DECL(_f64, double)
 ORACLE_DECL(_f32, float)

@@ -141,6 +141,13 @@ def lib():
             f.restype = C.c_int
             f.argtypes = ([C.POINTER(OracleSetObs), C.c_int, IP, IP, IP] + [P] * 5 + [IP, IP, P, IP]
                           + [IP] * 4 + [P] * 9 + [IP, IP])
+            f = getattr(_lib, "oracle_ensemble_mesp" + suf)
+            f.restype = None
+            f.argtypes = [C.c_int] * 5 + [ct] + [P] * 6
+            f = getattr(_lib, "oracle_grid_pack" + suf)
+            f.restype = None
+            FP = C.POINTER(C.c_float)
+            f.argtypes = [C.c_int] * 8 + [P] + [C.c_int64] * 3 + [P] + [C.c_int64] * 2 + [FP, FP]
     return _lib
 
 
@@ -175,6 +182,8 @@ def ref():
         _ref.ref_akima.restype = None
         _ref.ref_xtoy_consts.argtypes = [I, F]
         _ref.ref_xtoy_consts.restype = None
+        _ref.ref_ensemble_mesp.argtypes = [C.c_int] + [F] * 6
+        _ref.ref_ensemble_mesp.restype = None
     return _ref
 
 
@@ -468,3 +477,43 @@ def set_obs(c: OracleSetObs, elm, lon, lat, lev, dat, err, idx, idy, hx, qc=None
     res["nobs"] = nout
     res["ndel"] = (int(ndel[0]), int(ndel[1]))
     return res
+
+
+def ensemble_mesp(v3d, v2d, undef=999.0, dtype=np.float64):
+    """ensemble_mesp (common.f90:505-596): v3d (nij1,nlev,nmem,nv3d), v2d
+    (nij1,nmem,nv2d) in the Fortran layouts; returns v3dm, v3ds
+    (nij1,nlev,nv3d) and v2dm, v2ds (nij1,nv2d)."""
+    ct = _ct(dtype)
+    a3, a2 = _f(v3d, dtype), _f(v2d, dtype)
+    nij1, nlev, nmem, nv3d = a3.shape
+    nv2d = a2.shape[2]
+    if a2.shape[:2] != (nij1, nmem):
+        raise ValueError("v2d must be (nij1, nmem, nv2d)")
+    out = [np.zeros((nij1, nlev, nv3d), dtype, order="F") for _ in range(2)]
+    out += [np.zeros((nij1, nv2d), dtype, order="F") for _ in range(2)]
+    getattr(lib(), "oracle_ensemble_mesp" + _suffix(dtype))(nij1, nlev, nmem, nv3d, nv2d, undef, _ptr(a3, ct),
+                                                          _ptr(a2, ct), *(_ptr(o, ct) for o in out))
+    return tuple(out)
+
+
+def grid_pack(nlon, nlat, v3d, v2d, wrap=True, file_order=False, dtype=np.float64):
+    """Single-precision global grids of several fields as the output step
+    writes them (gather_grd_mpi on one rank, common_mpi.f90:133-206; the
+    quasi-global wrap, common_io.f90:960-963; the file's vertical order,
+    common_io.f90:655-661).  v3d (nij1,nlev,nfield,nv3d), v2d
+    (nij1,nfield,nv2d) in the Fortran layouts of the analysis arrays (the
+    fields being members); returns g3 (nlon,nlat,nlev,nv3d,nfield) and g2
+    (nlon,nlat,nv2d,nfield), Fortran order."""
+    ct = _ct(dtype)
+    a3, a2 = _f(v3d, dtype), _f(v2d, dtype)
+    nij, nlev, nf, nv3d = a3.shape
+    nv2d = a2.shape[2]
+    if nij != nlon * nlat or a2.shape[:2] != (nij, nf):
+        raise ValueError("v3d (nlon*nlat, nlev, nfield, nv3d), v2d (nlon*nlat, nfield, nv2d)")
+    g3 = np.zeros((nlon, nlat, nlev, nv3d, nf), np.float32, order="F")
+    g2 = np.zeros((nlon, nlat, nv2d, nf), np.float32, order="F")
+    getattr(lib(), "oracle_grid_pack" + _suffix(dtype))(
+        nlon, nlat, nlev, nv3d, nv2d, nf, int(wrap), int(file_order), _ptr(a3, ct), nij * nlev, nij * nlev * nf,
+        nij, _ptr(a2, ct), nij, nij * nf, _ptr(g3, C.c_float), _ptr(g2, C.c_float))
+    return g3, g2
+

@@ -1,16 +1,16 @@
 ! ref_driver.f90 -- build-owned bind(c) shim over the reference's own
 ! letkf_core (common_letkf.f90:61-210), mtx_eigen (common.f90:992-1050),
-! distance_2p (common.f90:79-101), akima_spline (common.f90:276-385) and
-! Trans_XtoY (common.f90:391-500), compiled unmodified from
-! /root/reference/letkf/run/src.  The reference fixes nmem = 128 and
+! distance_2p (common.f90:79-101), akima_spline (common.f90:276-385),
+! Trans_XtoY (common.f90:391-500) and ensemble_mesp (common.f90:505-596),
+! compiled unmodified from /root/reference/letkf/run/src.  The reference fixes nmem = 128 and
 ! r_size = kind(0.0e0) at common_setting.f90:20,36, so this library is the
 ! shipped FP32 build at K = 128.  TEST INFRASTRUCTURE ONLY.
 module letkf_ref_shim
   use iso_c_binding
   use common_setting, only: r_size, r_dble, nmem, pi, re, nlon, nlat, nlev, nv3d, nv2d, &
        lon, lat, depth, fsm, undef, hnosea, id_u_obs, id_v_obs, id_t_obs, id_s_obs, id_z_obs, &
-       iv3d_u, iv3d_v, iv3d_t, iv3d_s, iv2d_z
-  use common, only: distance_2p, mtx_eigen, akima_spline, Trans_XtoY
+       iv3d_u, iv3d_v, iv3d_t, iv3d_s, iv2d_z, nij1
+  use common, only: distance_2p, mtx_eigen, akima_spline, Trans_XtoY, ensemble_mesp
   use common_letkf, only: letkf_core
   implicit none
 contains
@@ -89,4 +89,15 @@ contains
     pi_out = pi
     re_out = re
   end subroutine ref_constants
+  ! ensemble_mesp on the reference's fixed nlev, nmem, nv3d and nv2d, with
+  ! the module's nij1 (common_setting.f90:168) set from the argument.
+  subroutine ref_ensemble_mesp(n, v3d, v2d, v3dm, v3ds, v2dm, v2ds) bind(c, name="ref_ensemble_mesp")
+    integer(c_int), value :: n
+    real(c_float), intent(in) :: v3d(n, nlev, nmem, nv3d), v2d(n, nmem, nv2d)
+    real(c_float), intent(out) :: v3dm(n, nlev, nv3d), v3ds(n, nlev, nv3d)
+    real(c_float), intent(out) :: v2dm(n, nv2d), v2ds(n, nv2d)
+    nij1 = n
+    call ensemble_mesp(v3d, v2d, v3dm, v3ds, v2dm, v2ds)
+  end subroutine ref_ensemble_mesp
+
 end module letkf_ref_shim

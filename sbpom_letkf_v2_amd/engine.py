@@ -330,6 +330,68 @@ def trans_xtoy_device(lon, lat, depth, fsm, elm, idx, idy, lono, lato, levo, v3d
     return hx, q
 
 
+def ensemble_mesp_device(v3d, v2d, undef=999.0):
+    """Grid ensemble mean and spread (letkf_ensemble_mesp_device;
+    ensemble_mesp, letkf/run/src/common.f90:505-596) of device tensors in the
+    analysis layout: v3d (nv3d, nmem, nlev, nij1) = v3d(nij1,nlev,nmem,nv3d),
+    v2d (nv2d, nmem, nij1).  Returns v3dm, v3ds (nv3d, nlev, nij1) and v2dm,
+    v2ds (nv2d, nij1)."""
+    import torch
+    nv3d, nmem, nlev, nij = v3d.shape
+    nv2d = v2d.shape[0]
+    dt = v3d.dtype
+    if v2d.shape[1:] != (nmem, nij) or v2d.dtype != dt or dt not in (torch.float32, torch.float64):
+        raise ValueError("v3d (nv3d, nmem, nlev, nij1), v2d (nv2d, nmem, nij1) of one real dtype")
+    for a in (v3d, v2d):
+        if not (a.is_cuda and a.is_contiguous()):
+            raise ValueError("v3d and v2d must be contiguous device tensors")
+    m3 = torch.empty((nv3d, nlev, nij), dtype=dt, device=v3d.device)
+    s3 = torch.empty_like(m3)
+    m2 = torch.empty((nv2d, nij), dtype=dt, device=v3d.device)
+    s2 = torch.empty_like(m2)
+    st = C.c_void_p(torch.cuda.current_stream(v3d.device).cuda_stream)
+    _lib.check(_lib.lib().letkf_ensemble_mesp_device(8 if dt == torch.float64 else 4, nij, nlev, nmem, nv3d,
+                                                      nv2d, float(undef), _ptr(v3d), _ptr(v2d), _ptr(m3),
+                                                      _ptr(s3), _ptr(m2), _ptr(s2), st))
+    return m3, s3, m2, s2
+
+
+def grid_pack_device(nlon, nlat, v3d, v2d, wrap=True, file_order=False, g3=None, g2=None):
+    """REAL(4) global grids of every field of v3d/v2d (letkf_grid_pack_device:
+    gather_grd_mpi on one rank, the quasi-global wrap, and with file_order
+    the file's vertical order; letkf/run/src/common_io.f90:863-1074).
+    v3d (nv3d, nfield, nlev, nlat*nlon) and v2d (nv2d, nfield, nlat*nlon) in
+    the analysis layout (fields = members), or (nv3d, nlev, nij1) / (nv2d,
+    nij1) for one field (a mean or a spread).  Returns g3 (nfield, nv3d,
+    nlev, nlat, nlon) and g2 (nfield, nv2d, nlat, nlon) float32, i.e.
+    v3dg(nlon,nlat,nlev,nv3d) / v2dg(nlon,nlat,nv2d) per field."""
+    import torch
+    one = v3d.dim() == 3
+    if one:
+        v3d, v2d = v3d.unsqueeze(1), v2d.unsqueeze(1)
+    nv3d, nf, nlev, nij = v3d.shape
+    nv2d = v2d.shape[0]
+    dt = v3d.dtype
+    if nij != nlon * nlat or v2d.shape[1:] != (nf, nij) or v2d.dtype != dt:
+        raise ValueError("v3d (nv3d, nfield, nlev, nlon*nlat), v2d (nv2d, nfield, nlon*nlat) of one dtype")
+    for a in (v3d, v2d):
+        if not (a.is_cuda and a.is_contiguous()):
+            raise ValueError("v3d and v2d must be contiguous device tensors")
+    if g3 is None:
+        g3 = torch.empty((nf, nv3d, nlev, nlat, nlon), dtype=torch.float32, device=v3d.device)
+    if g2 is None:
+        g2 = torch.empty((nf, nv2d, nlat, nlon), dtype=torch.float32, device=v3d.device)
+    if g3.shape != (nf, nv3d, nlev, nlat, nlon) or g2.shape != (nf, nv2d, nlat, nlon):
+        raise ValueError("g3/g2 shapes")
+    s3 = (C.c_int64 * 3)(nij * nlev, nij * nlev * nf, nij)
+    s2 = (C.c_int64 * 2)(nij, nij * nf)
+    st = C.c_void_p(torch.cuda.current_stream(v3d.device).cuda_stream)
+    _lib.check(_lib.lib().letkf_grid_pack_device(8 if dt == torch.float64 else 4, nlon, nlat, nlev, nv3d, nv2d,
+                                                  nf, int(wrap), int(file_order), _ptr(v3d), s3, _ptr(v2d), s2,
+                                                  _ptr(g3), _ptr(g2), st))
+    return g3, g2
+
+
 def da_main(pr, precision="fp64", form="plain"):
     """Analysis of one problem; returns (anal3d, anal2d) in the same kind
     of container as the inputs."""

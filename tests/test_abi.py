@@ -114,6 +114,13 @@ def test_argument_errors_are_status_codes():
     xg = _lib.XtoyGrid(nlon=8, nlat=6, nlev=3, nv3d=4, nv2d=1, real_kind=8)
     assert L.letkf_trans_xtoy_device(C.byref(xg), 1, *([None] * 6), 2, None, None, None, None, None,
                                      None) == -1
+    # the output-step entries check kinds and sizes before any HIP call
+    assert L.letkf_ensemble_mesp_device(3, 10, 2, 4, 1, 1, 999.0, *([None] * 7)) == -1
+    assert L.letkf_ensemble_mesp_device(8, 10, 2, 0, 1, 1, 999.0, *([None] * 7)) == -1
+    assert L.letkf_ensemble_mesp_device(8, 0, 2, 4, 1, 1, 999.0, *([None] * 7)) == 0
+    assert L.letkf_grid_pack_device(8, 2, 10, 3, 1, 1, 1, 1, 0, *([None] * 7)) == -1   # wrap, nlon < 4
+    assert L.letkf_grid_pack_device(8, 8, 10, 3, 1, 1, 1, 0, 0, *([None] * 7)) == -1   # NULL arrays
+    assert L.letkf_grid_pack_device(8, 8, 10, 3, 1, 1, 0, 1, 0, *([None] * 7)) == 0    # no fields
     with pytest.raises(_lib.EngineError):
         _lib.check(-1)
 
