@@ -8,9 +8,11 @@
 //    write_state_vector_all (common_io.f90:655-661).  With the wrap and in
 //    model order the same grids are what Trans_XtoY reads for H(x^a)
 //    (common_io.f90:966-989), so letkf_trans_xtoy_device takes them as is.
-// Both are streaming kernels: one thread per grid value, consecutive
-// threads on consecutive values (coalesced), member loops serial in member
-// order as in the reference so that results are bit-identical.
+// Both are streaming kernels with consecutive threads on consecutive grid
+// values (coalesced): k_mesp_reg holds a grid value's members in registers
+// (up to 128; k_mesp reads them twice beyond), its sums run in member order
+// as in the reference so that results are bit-identical; k_grid_pack
+// strides along longitude rows.
 #include "letkf_handle.h"
 
 using namespace letkf;
@@ -83,35 +85,114 @@ __global__ void __launch_bounds__(256) k_mesp(int nij1, int nlev, int nmem, int 
   }
 }
 
+// nmem <= NM: the members are loaded once into registers (all loads in
+// flight together) and both sums run over the registers; same operations in
+// the same order as k_mesp.
+template <typename T, int NM>
+__global__ void __launch_bounds__(256) k_mesp_reg(int nij1, int nlev, int nmem, int nv3d, int nv2d, T undef,
+                                                  const T* __restrict__ v3d, const T* __restrict__ v2d,
+                                                  T* __restrict__ v3dm, T* __restrict__ v3ds, T* __restrict__ v2dm,
+                                                  T* __restrict__ v2ds) {
+#pragma clang fp contract(off)
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nij1) return;
+  const long long ni = nij1;
+  const int rows = nv3d * nlev + nv2d;
+  for (int r = blockIdx.y; r < rows; r += gridDim.y) {
+    const T* src;
+    long long ms;
+    T *om, *os;
+    if (r < nv3d * nlev) {
+      const int v = r / nlev, k = r - v * nlev;
+      src = v3d + ni * (k + (long long)nlev * nmem * v) + i;
+      ms = ni * nlev;
+      om = v3dm + ni * (k + (long long)nlev * v) + i;
+      os = v3ds + ni * (k + (long long)nlev * v) + i;
+    } else {
+      const int v = r - nv3d * nlev;
+      src = v2d + ni * nmem * v + i;
+      ms = ni;
+      om = v2dm + ni * v + i;
+      os = v2ds + ni * v + i;
+    }
+    T x[NM];
+#pragma unroll
+    for (int u = 0; u < NM; ++u) x[u] = u < nmem ? src[ms * u] : (T)0;
+    T s = (T)0;
+#pragma unroll
+    for (int u = 0; u < NM; ++u)
+      if (u < nmem) s = s + x[u];
+    s = s / (T)nmem;
+    *om = s;
+    T q = (T)0;
+#pragma unroll
+    for (int u = 0; u < NM; ++u)
+      if (u < nmem) {
+        const T d = x[u] - s;
+        q = q + d * d;
+      }
+    *os = nmem <= 1 ? undef : sqrt_t<T>(q / (T)(nmem - 1));
+  }
+}
+
+template <typename T>
+void launch_mesp(dim3 grid, hipStream_t s, int nij1, int nlev, int nmem, int nv3d, int nv2d, T undef, const T* v3d,
+                 const T* v2d, T* v3dm, T* v3ds, T* v2dm, T* v2ds) {
+#define MESP_ARGS s, nij1, nlev, nmem, nv3d, nv2d, undef, v3d, v2d, v3dm, v3ds, v2dm, v2ds
+  if (nmem <= 16)
+    hipLaunchKernelGGL((k_mesp_reg<T, 16>), grid, dim3(256), 0, MESP_ARGS);
+  else if (nmem <= 32)
+    hipLaunchKernelGGL((k_mesp_reg<T, 32>), grid, dim3(256), 0, MESP_ARGS);
+  else if (nmem <= 64)
+    hipLaunchKernelGGL((k_mesp_reg<T, 64>), grid, dim3(256), 0, MESP_ARGS);
+  else if (nmem <= 128)
+    hipLaunchKernelGGL((k_mesp_reg<T, 128>), grid, dim3(256), 0, MESP_ARGS);
+  else
+    hipLaunchKernelGGL(k_mesp<T>, grid, dim3(256), 0, MESP_ARGS);
+#undef MESP_ARGS
+}
+
 struct PackArgs {
   int nlon, nlat, nlev, nv3d, nv2d, nfield, wrap, file_order;
   long long f3, s3v, s3k, f2, s2v;
 };
 
 // Output row r of field f: rows < nv3d nlev are (variable, output level),
-// the rest 2-D variables.  Output value i = (ilon, ilat) reads the local
-// value i, or i - (nlon - 2) in the last two columns under the wrap.
+// the rest 2-D variables.  A workgroup covers kPackRows latitude rows of
+// one output row, its threads striding along longitude (no division per
+// value); value (ilon, ilat) reads local value (ilon, ilat), or (ilon -
+// (nlon - 2), ilat) in the last two columns under the wrap.
+constexpr int kPackRows = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_grid_pack(PackArgs a, const T* __restrict__ v3d, const T* __restrict__ v2d,
                                                    float* __restrict__ g3, float* __restrict__ g2) {
   const long long nij = (long long)a.nlon * a.nlat;
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= nij) return;
-  const int ilon = (int)(i % a.nlon);
-  const long long si = a.wrap && ilon >= a.nlon - 2 ? i - (a.nlon - 2) : i;
   const int per = a.nv3d * a.nlev + a.nv2d;
   const long long rows = (long long)per * a.nfield;
+  const int j0 = blockIdx.x * kPackRows;
+  const int j1 = min(j0 + kPackRows, a.nlat);
+  const int wcol = a.wrap ? a.nlon - 2 : a.nlon;  // first wrapped column
   for (long long rr = blockIdx.y; rr < rows; rr += gridDim.y) {
     const int f = (int)(rr / per), r = (int)(rr - (long long)f * per);
+    const T* src;
+    float* dst;
     if (r < a.nv3d * a.nlev) {
       const int v = r / a.nlev, ko = r - v * a.nlev;
       const int k = a.file_order ? a.nlev - 1 - ko : ko;
-      const T x = v3d[a.f3 * f + a.s3v * v + a.s3k * k + si];
-      g3[nij * ((long long)a.nlev * a.nv3d * f + (long long)a.nlev * v + ko) + i] = (float)x;
+      src = v3d + a.f3 * f + a.s3v * v + a.s3k * k;
+      dst = g3 + nij * ((long long)a.nlev * a.nv3d * f + (long long)a.nlev * v + ko);
     } else {
       const int v = r - a.nv3d * a.nlev;
-      const T x = v2d[a.f2 * f + a.s2v * v + si];
-      g2[nij * ((long long)a.nv2d * f + v) + i] = (float)x;
+      src = v2d + a.f2 * f + a.s2v * v;
+      dst = g2 + nij * ((long long)a.nv2d * f + v);
+    }
+    for (int j = j0; j < j1; ++j) {
+      const long long row = (long long)a.nlon * j;
+      for (int ilon = threadIdx.x; ilon < a.nlon; ilon += 256) {
+        const int sc = ilon >= wcol ? ilon - wcol : ilon;
+        dst[row + ilon] = (float)src[row + sc];
+      }
     }
   }
 }
@@ -140,13 +221,11 @@ extern "C" int letkf_ensemble_mesp_device(int32_t real_kind, int32_t nij1, int32
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)((nij1 + 255) / 256), (unsigned)std::min(rows, 65535));
   if (real_kind == 8)
-    hipLaunchKernelGGL(k_mesp<double>, grid, dim3(256), 0, s, nij1, nlev, nmem, nv3d, nv2d, undef,
-                       (const double*)v3d, (const double*)v2d, (double*)v3dm, (double*)v3ds, (double*)v2dm,
-                       (double*)v2ds);
+    launch_mesp<double>(grid, s, nij1, nlev, nmem, nv3d, nv2d, undef, (const double*)v3d, (const double*)v2d,
+                        (double*)v3dm, (double*)v3ds, (double*)v2dm, (double*)v2ds);
   else
-    hipLaunchKernelGGL(k_mesp<float>, grid, dim3(256), 0, s, nij1, nlev, nmem, nv3d, nv2d, (float)undef,
-                       (const float*)v3d, (const float*)v2d, (float*)v3dm, (float*)v3ds, (float*)v2dm,
-                       (float*)v2ds);
+    launch_mesp<float>(grid, s, nij1, nlev, nmem, nv3d, nv2d, (float)undef, (const float*)v3d, (const float*)v2d,
+                       (float*)v3dm, (float*)v3ds, (float*)v2dm, (float*)v2ds);
   const hipError_t e = finish(s);
   if (e != hipSuccess) return set_err(LETKF_ERR_HIP, "letkf_ensemble_mesp_device: %s", hipGetErrorString(e));
   return LETKF_OK;
@@ -170,8 +249,7 @@ extern "C" int letkf_grid_pack_device(int32_t real_kind, int32_t nlon, int32_t n
   PackArgs a{nlon, nlat, nlev, nv3d, nv2d, nfield, wrap ? 1 : 0, file_order ? 1 : 0,
              s3 ? s3[0] : 0, s3 ? s3[1] : 0, s3 ? s3[2] : 0, s2 ? s2[0] : 0, s2 ? s2[1] : 0};
   hipStream_t s = (hipStream_t)stream;
-  const long long nij = (long long)nlon * nlat;
-  const dim3 grid((unsigned)((nij + 255) / 256), (unsigned)std::min<long long>(rows, 65535));
+  const dim3 grid((unsigned)((nlat + kPackRows - 1) / kPackRows), (unsigned)std::min<long long>(rows, 65535));
   if (real_kind == 8)
     hipLaunchKernelGGL(k_grid_pack<double>, grid, dim3(256), 0, s, a, (const double*)v3d, (const double*)v2d, g3,
                        g2);
