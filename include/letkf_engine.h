@@ -354,6 +354,26 @@ int letkf_set_obs_device(letkf_handle_t* h, const letkf_setobs_cfg_t* cfg,
                          const void* hx, const int32_t* qc,
                          letkf_obs_set_t* out);
 
+/* lpfgm_core (common_lpfgm.f90:444-494) batched over nb local problems in
+ * the CSR form of letkf_core_batched (DEVICE offs, hdxf, rdiag, rloc, dep;
+ * outputs wvec (nb,nmem) = 0 and wmat (nb,nmem,nmem), wmat[b][i][j] =
+ * Wmat(i,j)): mode 2 = lpf_core (213-263), the average of nmonte resampling
+ * matrices (resampling_matrix, 89-197); mode 3 = gm_core (279-405) times
+ * lpf_core, gm_core's alpha = (nmem-1)/cov_infl_gm.  The reference draws
+ * each resampling's nmem uniforms from MT19937 seeded by the wall clock
+ * (com_rand_seed(nmem,0,.), common.f90:1066-1095); here the caller passes
+ * them: rand (HOST, nmonte x nmem doubles in [0,1), resampling m in row m),
+ * used by every problem.  rm_sr 0 = "MR" (draws sorted ascending), 1 =
+ * "SR" (rand(i) = (rand(1) + i - 1)/nmem in place, as the reference
+ * writes it); dp = diagonal priority.  nmem <= 128, nmonte <= 255.  A
+ * problem with no rows gets lpfgm_core_noobs (W = I). */
+int letkf_lpfgm_core_batched(letkf_handle_t* h, int32_t mode,
+                             double cov_infl_gm, int32_t nmonte, int32_t rm_sr,
+                             int32_t dp, const double* rand, int32_t nb,
+                             const int64_t* offs, const void* hdxf,
+                             const void* rdiag, const void* rloc,
+                             const void* dep, void* wvec, void* wmat);
+
 /* ensemble_mesp (common.f90:505-596) on DEVICE arrays, as write_ens_mpi
  * calls it for the analysis and the forecast (common_io.f90:922-924):
  * v3d(nij1,nlev,nmem,nv3d), v2d(nij1,nmem,nv2d) in, v3dm/v3ds

@@ -130,7 +130,12 @@ typedef struct oracle_tables_f32 { ORACLE_TABLES(float) } oracle_tables_f32_t;
                              int nfield, int wrap, int file_order,              \
                              const REAL* v3d, int64_t f3, int64_t s3v,          \
                              int64_t s3k, const REAL* v2d, int64_t f2,          \
-                             int64_t s2v, float* g3, float* g2);
+                             int64_t s2v, float* g3, float* g2);              \
+  int oracle_lpfgm_core##SUF(int n, int K, int mode, double cov_infl_gm,        \
+                             int nmonte, int rm_sr, int dp,                     \
+                             const double* rand, const REAL* hdxf,              \
+                             const REAL* rdiag, const REAL* rloc,               \
+                             const REAL* dep, REAL* wvec, REAL* wmat);
 
 ORACLE_DECL(_f64, double)
 ORACLE_DECL(_f32, float)

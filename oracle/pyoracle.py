@@ -144,6 +144,9 @@ def lib():
             f = getattr(_lib, "oracle_ensemble_mesp" + suf)
             f.restype = None
             f.argtypes = [C.c_int] * 5 + [ct] + [P] * 6
+            f = getattr(_lib, "oracle_lpfgm_core" + suf)
+            f.restype = C.c_int
+            f.argtypes = [C.c_int] * 3 + [C.c_double] + [C.c_int] * 3 + [C.POINTER(C.c_double)] + [P] * 6
             f = getattr(_lib, "oracle_grid_pack" + suf)
             f.restype = None
             FP = C.POINTER(C.c_float)
@@ -516,4 +519,24 @@ def grid_pack(nlon, nlat, v3d, v2d, wrap=True, file_order=False, dtype=np.float6
         nlon, nlat, nlev, nv3d, nv2d, nf, int(wrap), int(file_order), _ptr(a3, ct), nij * nlev, nij * nlev * nf,
         nij, _ptr(a2, ct), nij, nij * nf, _ptr(g3, C.c_float), _ptr(g2, C.c_float))
     return g3, g2
+
+
+def lpfgm_core(hdxf, rdiag, rloc, dep, rand, mode=3, cov_infl_gm=1.0, rm="MR", dp=True, dtype=np.float64):
+    """lpfgm_core (common_lpfgm.f90:444-494): mode 2 = LPF, 3 = LPFGM.
+    hdxf (nobsl, nmem); rand (nmonte, nmem) uniforms in [0, 1), the draws of
+    the nmonte resamplings.  Returns rc, wvec (nmem), Wmat (nmem, nmem)."""
+    ct = _ct(dtype)
+    hdxf = _f(hdxf, dtype)
+    n, K = hdxf.shape
+    rdiag, rloc, dep = (_f(x, dtype) for x in (rdiag, rloc, dep))
+    r = np.ascontiguousarray(rand, np.float64)
+    if r.ndim != 2 or r.shape[1] != K:
+        raise ValueError("rand must be (nmonte, nmem)")
+    w = np.zeros(K, dtype)
+    W = np.zeros((K, K), dtype, order="F")
+    rc = getattr(lib(), "oracle_lpfgm_core" + _suffix(dtype))(
+        n, K, mode, cov_infl_gm, r.shape[0], 1 if rm == "SR" else 0, 1 if dp else 0,
+        r.ctypes.data_as(C.POINTER(C.c_double)), _ptr(hdxf, ct), _ptr(rdiag, ct), _ptr(rloc, ct), _ptr(dep, ct),
+        _ptr(w, ct), _ptr(W, ct))
+    return rc, w, W
 

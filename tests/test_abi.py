@@ -114,6 +114,7 @@ def test_argument_errors_are_status_codes():
     xg = _lib.XtoyGrid(nlon=8, nlat=6, nlev=3, nv3d=4, nv2d=1, real_kind=8)
     assert L.letkf_trans_xtoy_device(C.byref(xg), 1, *([None] * 6), 2, None, None, None, None, None,
                                      None) == -1
+    assert L.letkf_lpfgm_core_batched(None, 3, 1.0, 100, 0, 1, None, 0, *([None] * 7)) == -1
     # the output-step entries check kinds and sizes before any HIP call
     assert L.letkf_ensemble_mesp_device(3, 10, 2, 4, 1, 1, 999.0, *([None] * 7)) == -1
     assert L.letkf_ensemble_mesp_device(8, 10, 2, 0, 1, 1, 999.0, *([None] * 7)) == -1
