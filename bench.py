@@ -64,6 +64,10 @@ def parse():
                     help="time the device half of the output step (default workload c1): grid mean/spread "
                          "of analysis and forecast and the REAL(4) file images of every field, with a "
                          "bit-exact oracle check and CPU baseline on two levels")
+    ap.add_argument("--lpfgm", action="store_true",
+                    help="time the LPFGM core (letkf_lpfgm_core_batched) on local problems of nmem 100 and 84 "
+                         "obs, with an oracle check and CPU baseline on a sample")
+    ap.add_argument("--lpfgm-problems", type=int, default=50000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise kernel that reads and writes 1 GiB (three launches), the "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -348,6 +352,85 @@ def run_output(args):
                         "unit": "GB/s", "frac": gbs(b_mesp + b_pack, ms) / PEAK_HBM_GBS, "traffic": None},
            "kernel_roofline": {"k_mesp": {"ms": ms_mesp, "achieved": gbs(b_mesp, ms_mesp), "unit": "GB/s"},
                                "k_grid_pack": {"ms": ms_pack, "achieved": gbs(b_pack, ms_pack), "unit": "GB/s"}},
+           "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+def run_lpfgm(args):
+    """The local particle filter and its Gaussian-mixture extension
+    (lpfgm_core, common_lpfgm.f90:444-494) batched over local problems of
+    the north-star shape (nmem 100, 84 local obs each, the mean count of the
+    c1 analysis), 100 resamplings per problem: one step = one
+    letkf_lpfgm_core_batched call (mode 3, LPFGM).  Work figure: the Gram
+    2 K^2 n, the eigensolve (4/3) K^3 for its reduction + ~6 K^3 for its
+    vectors, W_GM 2 K^3 and the product W_GM W_LPF 2 K^3 flop per problem,
+    FP64 on the vector units."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import pyoracle as po
+    from sbpom_letkf_v2_amd import _lib, engine, synth
+    K, nobsl, nmonte, nb = 100, 84, 100, args.lpfgm_problems
+    rng = np.random.default_rng(5)
+    offs = (np.arange(nb + 1) * nobsl).astype(np.int64)
+    Y = rng.standard_normal((nb * nobsl, K)) * 0.6
+    Y -= Y.mean(1, keepdims=True)
+    rd = rng.uniform(0.5, 2.0, nb * nobsl) ** 2
+    rl = rng.uniform(0.05, 1.0, nb * nobsl)
+    dep = rng.standard_normal(nb * nobsl) * 0.6
+    draws = np.ascontiguousarray(rng.uniform(size=(nmonte, K)))
+    pr = synth.make_problem(nlon=8, nlat=6, nlev=2, nmem=K, nobs=10, seed=1)
+    eng = engine.Engine(engine.make_params(pr))
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    tO, tY, trd, trl, tdp = dev(offs), dev(Y), dev(rd), dev(rl), dev(dep)
+    w = torch.empty((nb, K), dtype=torch.float64, device="cuda")
+    W = torch.empty((nb, K, K), dtype=torch.float64, device="cuda")
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    rp = draws.ctypes.data_as(C.POINTER(C.c_double))
+    call = lambda mode: _lib.check(_lib.lib().letkf_lpfgm_core_batched(
+        eng.h, mode, 1.0, nmonte, 0, 1, rp, nb, vp(tO), vp(tY), vp(trd), vp(trl), vp(tdp), vp(w), vp(W)))
+    call(2)   # LPF alone first (warm), then the timed LPFGM steps leave W
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    call(2)
+    torch.cuda.synchronize()
+    ms_lpf = (time.perf_counter() - t0) * 1e3
+    for _ in range(args.warmup):
+        call(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call(3)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    flop = nb * (2.0 * K * K * nobsl + (4.0 / 3.0 + 6.0 + 4.0) * K ** 3)
+    tf = flop / (ms * 1e-3) / 1e12
+    log(f"  LPFGM {ms:.1f} ms per {nb} problems ({nb / (ms * 1e-3):.3e} problems/s, {tf:.2f} TF/s FP64); "
+        f"LPF alone {ms_lpf:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        Wg = W.cpu().numpy()
+        ns = min(nb, 400)
+        t0 = time.perf_counter()
+        err = 0.0
+        for b in range(ns):
+            seg = slice(offs[b], offs[b + 1])
+            rc, _, Wo = po.lpfgm_core(Y[seg], rd[seg], rl[seg], dep[seg], draws, mode=3, cov_infl_gm=1.0)
+            err = max(err, float(np.abs(Wg[b] - Wo).max() / np.abs(Wo).max()))
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": ns / t_cpu, "unit": "problems/s", "cores": 1, "kind": "port",
+               "sample": f"{ns} of {nb} problems, FP64 C oracle (one thread), {t_cpu:.2f} s"}
+        log(f"  oracle on {ns} problems: max rel err {err:.2e}; cpu {cpu['value']:.1f} problems/s")
+        if err > 1e-10:
+            raise SystemExit("LPFGM differs from the oracle")
+    eng.close()
+    out = {"metric": "LPFGM local problems per second", "value": nb / (ms * 1e-3), "unit": "problems/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{nb} local problems, nmem {K}, {nobsl} obs each, nmonte {nmonte}, MR + DP"},
+           "stage_ms": {"lpfgm": ms, "lpf_only": ms_lpf},
+           "roofline": {"bound": "mfma", "achieved": tf, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                        "frac": tf / PEAK_TFLOPS["fp64"], "traffic": None},
            "cpu_baseline": cpu}
     print(json.dumps(out))
 
@@ -669,6 +752,8 @@ def main():
         return run_setobs(args)
     if args.output:
         return run_output(args)
+    if args.lpfgm:
+        return run_lpfgm(args)
     if args.xtoy:
         run_xtoy(args)
         return

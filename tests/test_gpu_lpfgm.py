@@ -71,7 +71,7 @@ def _run(K, sizes, mode, rm, dp, dtype, nmonte=100, gamma=1.3, seed=0):
     return out
 
 
-@pytest.mark.parametrize("K", [20, 64, 128])
+@pytest.mark.parametrize("K", [20, 64, 100, 128])
 @pytest.mark.parametrize("rm,dp", [("MR", True), ("SR", True), ("MR", False)])
 def test_lpf_bit_exact(K, rm, dp):
     _say(f"case lpf K={K} {rm} dp={dp}")
@@ -79,7 +79,7 @@ def test_lpf_bit_exact(K, rm, dp):
         np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("K", [20, 64, 128])
+@pytest.mark.parametrize("K", [20, 64, 100, 128])
 def test_lpfgm_vs_oracle(K):
     _say(f"case lpfgm K={K}")
     for got, want in _run(K, [0, 1, 3, K - 1, K, 2 * K + 5, 17, 250], 3, "MR", True, np.float64):

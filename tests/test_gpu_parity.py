@@ -132,7 +132,7 @@ def _csr_problems(rng, K, sizes, dtype):
     return offs, [np.asarray(a, dtype) for a in (Y, rd, rl, dp)]
 
 
-@pytest.mark.parametrize("K", [20, 64, 128])
+@pytest.mark.parametrize("K", [20, 64, 100, 128])
 def test_core_batched_vs_oracle(K):
     """letkf_core_batched (wvec, Wmat of common_letkf.f90:37-210) vs the
     FP64 oracle's letkf_core per problem, including empty problems."""
