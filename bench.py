@@ -361,10 +361,10 @@ def run_lpfgm(args):
     (lpfgm_core, common_lpfgm.f90:444-494) batched over local problems of
     the north-star shape (nmem 100, 84 local obs each, the mean count of the
     c1 analysis), 100 resamplings per problem: one step = one
-    letkf_lpfgm_core_batched call (mode 3, LPFGM).  Work figure: the Gram
-    2 K^2 n, the eigensolve (4/3) K^3 for its reduction + ~6 K^3 for its
-    vectors, W_GM 2 K^3 and the product W_GM W_LPF 2 K^3 flop per problem,
-    FP64 on the vector units."""
+    letkf_lpfgm_core_batched call (mode 3, LPFGM).  Work figure per problem:
+    the Gram 2 K^2 n, the Cholesky factor K^3 / 3, its inverse K^3 / 3,
+    Pa = L^-T L^-1 K^3 / 3 and the product W_GM W_LPF 2 K^3 flop, FP64 on
+    the vector units."""
     import ctypes as C
     import numpy as np
     import torch
@@ -403,7 +403,7 @@ def run_lpfgm(args):
         call(3)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    flop = nb * (2.0 * K * K * nobsl + (4.0 / 3.0 + 6.0 + 4.0) * K ** 3)
+    flop = nb * (2.0 * K * K * nobsl + 3.0 * K ** 3)
     tf = flop / (ms * 1e-3) / 1e12
     log(f"  LPFGM {ms:.1f} ms per {nb} problems ({nb / (ms * 1e-3):.3e} problems/s, {tf:.2f} TF/s FP64); "
         f"LPF alone {ms_lpf:.1f} ms")

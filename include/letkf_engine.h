@@ -359,7 +359,8 @@ int letkf_set_obs_device(letkf_handle_t* h, const letkf_setobs_cfg_t* cfg,
  * outputs wvec (nb,nmem) = 0 and wmat (nb,nmem,nmem), wmat[b][i][j] =
  * Wmat(i,j)): mode 2 = lpf_core (213-263), the average of nmonte resampling
  * matrices (resampling_matrix, 89-197); mode 3 = gm_core (279-405) times
- * lpf_core, gm_core's alpha = (nmem-1)/cov_infl_gm.  The reference draws
+ * lpf_core, gm_core's alpha = (nmem-1)/cov_infl_gm (its Pa = C^-1 from a
+ * Cholesky factorisation rather than mtx_eigen).  The reference draws
  * each resampling's nmem uniforms from MT19937 seeded by the wall clock
  * (com_rand_seed(nmem,0,.), common.f90:1066-1095); here the caller passes
  * them: rand (HOST, nmonte x nmem doubles in [0,1), resampling m in row m),

@@ -17,10 +17,12 @@
 //    matrices is a table lookup of the count (the running sum of 1/nmonte in
 //    the real kind, as the reference accumulates it);
 //  * LPFGM (lpfgm_core, :444-494): W_GM (gm_core, :279-405) in its closed
-//    form alpha Pa + (Pa b) 1^T from the eigen-decomposition of
-//    C = Y^T R^-1 Y + alpha I (alpha = (m-1)/cov_infl_gm; the Gram and the
-//    FP64 eigensolve of letkf_core_batched), then W = W_GM W_LPF, W_GM rows
-//    staged through LDS.
+//    form alpha Pa + (Pa b) 1^T, Pa = C^-1, C = Y^T R^-1 Y + alpha I
+//    (alpha = (m-1)/cov_infl_gm; the Gram of letkf_core_batched).  The
+//    reference forms Pa from mtx_eigen; C >= alpha I is well conditioned, so
+//    the workgroup inverts it through its Cholesky factor in LDS instead (the
+//    same Pa to FP64 rounding, a tenth of the eigensolve's time).  Then
+//    W = W_GM W_LPF, W_GM rows staged through the same LDS.
 #include <algorithm>
 #include <vector>
 
@@ -38,13 +40,12 @@ __global__ void __launch_bounds__(256) k_lpfgm(int K, int mode, int nmonte, int 
                                                const int64_t* __restrict__ offs, const T* __restrict__ hdxf,
                                                const T* __restrict__ rdiag, const T* __restrict__ rloc,
                                                const T* __restrict__ dep, const double* __restrict__ rtab,
-                                               const double* __restrict__ ttab, const double* __restrict__ lam,
-                                               const double* __restrict__ V, const double* __restrict__ bvec,
-                                               T* __restrict__ wvec, T* __restrict__ wmat) {
+                                               const double* __restrict__ ttab, const double* __restrict__ G,
+                                               const double* __restrict__ bvec, T* __restrict__ wvec,
+                                               T* __restrict__ wmat) {
 #pragma clang fp contract(off)
   __shared__ double ll[kLpfMaxK], cw[kLpfMaxK], sq[kLpfMaxK], wb[kLpfMaxK], il[kLpfMaxK];
   __shared__ uint32_t cnt[kLpfMaxK * kLpfMaxK / 4];  // byte count per (particle, column)
-  __shared__ double gm[kLpfRows][kLpfMaxK];
   __shared__ double tt[256];
   const int b = blockIdx.x, t = threadIdx.x;
   const long long o0 = offs[b], o1 = offs[b + 1];
@@ -140,39 +141,76 @@ __global__ void __launch_bounds__(256) k_lpfgm(int K, int mode, int nmonte, int 
     for (int q = t; q < K * K; q += 256) Wb[q] = (T)wlpf(q / K, q % K);
     return;
   }
-  // ---- W_GM = alpha Pa + (Pa b) 1^T, Pa = V lam^-1 V^T ----
-  const double* Vb = V + (long long)b * K * K;
-  const double* lb = lam + (long long)b * K;
+  // ---- W_GM = alpha Pa + (Pa b) 1^T with Pa = C^-1 = L^-T L^-1 ----
+  // C (upper triangle in G) -> LDS, lower Cholesky factor L in place
+  extern __shared__ double cm[];  // K x K, row pitch K
+  const double* Gb = G + (long long)b * K * K;
+  for (int q = t; q < K * K; q += 256) {
+    const int i = q / K, j = q - i * K;
+    if (j <= i) cm[q] = Gb[j * K + i];
+  }
+  __syncthreads();
+  for (int j = 0; j < K; ++j) {
+    if (t == 0) {
+      const double d = cm[j * K + j];
+      cm[j * K + j] = d > 0.0 ? sqrt(d) : NAN;
+    }
+    __syncthreads();
+    const double ljj = cm[j * K + j];
+    for (int i = j + 1 + t; i < K; i += 256) cm[i * K + j] /= ljj;
+    __syncthreads();
+    const int m = K - j - 1;  // trailing block rows/cols j+1..K-1, lower part
+    for (int q = t; q < m * m; q += 256) {
+      const int i = j + 1 + q / m, k = j + 1 + q % m;
+      if (k <= i) cm[i * K + k] -= cm[i * K + j] * cm[k * K + j];
+    }
+    __syncthreads();
+  }
+  // L^-1: column j by lane j, stored transposed in the upper triangle
+  // (Linv(i,j) at cm[j*K + i], i > j); its diagonal in il
+  if (t < K) il[t] = 1.0 / cm[t * K + t];
+  __syncthreads();
+  if (t < K) {
+    const int j = t;
+    for (int i = j + 1; i < K; ++i) {
+      double acc = cm[i * K + j] * il[j];
+      for (int k = j + 1; k < i; ++k) acc = fma(cm[i * K + k], cm[j * K + k], acc);
+      cm[j * K + i] = -acc * il[i];
+    }
+  }
+  __syncthreads();
+  auto linv = [&](int i, int j) -> double { return i == j ? il[i] : cm[j * K + i]; };  // i >= j
+  // w = Pa b = L^-T (L^-1 b)
   const double* bb = bvec + (long long)b * K;
   if (t < K) {
     double acc = 0.0;
-    for (int r = 0; r < K; ++r) acc = fma(Vb[r * K + t], bb[r], acc);
-    il[t] = 1.0 / lb[t];
-    sq[t] = acc / lb[t];
+    for (int m2 = 0; m2 <= t; ++m2) acc = fma(linv(t, m2), bb[m2], acc);
+    sq[t] = acc;
   }
   __syncthreads();
   if (t < K) {
     double acc = 0.0;
-    for (int c = 0; c < K; ++c) acc = fma(Vb[t * K + c], sq[c], acc);
+    for (int k = t; k < K; ++k) acc = fma(linv(k, t), sq[k], acc);
     wb[t] = acc;
   }
   __syncthreads();
   for (int q = t; q < K * K; q += 256) {
     const int i = q / K, j = q - i * K;
+    const int k0 = i > j ? i : j;
     double acc = 0.0;
-    for (int c = 0; c < K; ++c) acc = fma(Vb[i * K + c] * il[c], Vb[j * K + c], acc);
+    for (int k = k0; k < K; ++k) acc = fma(linv(k, i), linv(k, j), acc);
     Wb[q] = (T)(alpha * acc + wb[i]);
   }
   __syncthreads();
-  // ---- W = W_GM W_LPF, kLpfRows rows of W_GM at a time ----
+  // ---- W = W_GM W_LPF, kLpfRows rows of W_GM at a time (staged in cm) ----
   for (int i0 = 0; i0 < K; i0 += kLpfRows) {
     const int nr = min(kLpfRows, K - i0);
-    for (int q = t; q < nr * K; q += 256) gm[q / K][q % K] = (double)Wb[(long long)(i0 + q / K) * K + q % K];
+    for (int q = t; q < nr * K; q += 256) cm[q] = (double)Wb[(long long)i0 * K + q];
     __syncthreads();
     for (int q = t; q < nr * K; q += 256) {
       const int i = q / K, k = q - i * K;
       double acc = 0.0;
-      for (int j = 0; j < K; ++j) acc = fma(gm[i][j], wlpf(j, k), acc);
+      for (int j = 0; j < K; ++j) acc = fma(cm[i * K + j], wlpf(j, k), acc);
       Wb[(long long)(i0 + i) * K + k] = (T)acc;
     }
     __syncthreads();
@@ -218,17 +256,13 @@ extern "C" int letkf_lpfgm_core_batched(letkf_handle_t* h, int32_t mode, double 
   double* d_tt = d_rt + rt.size();
   HIP_TRY(hipMemcpyAsync(d_rt, rt.data(), sizeof(double) * rt.size(), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d_tt, tt.data(), sizeof(double) * tt.size(), hipMemcpyHostToDevice, s));
-  const double* lam = nullptr;
-  const double* Vp = nullptr;
+  const double* Gp = nullptr;
   const double* bv = nullptr;
   double alpha = 0.0;
   if (mode == 3) {
     HIP_TRY(h->G.ensure(sizeof(double) * (size_t)nb * K * K));
-    HIP_TRY(h->V.ensure(sizeof(double) * (size_t)nb * K * K));
-    HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)nb * K));
     HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)nb * K));
     HIP_TRY(h->nrow.ensure(sizeof(int32_t) * (size_t)nb));
-    HIP_TRY(hipMemsetAsync(h->status.ptr, 0, sizeof(int) * 4, s));
     if (f64)
       HIP_TRY(launch_gram_csr<double>(K, cov_infl_gm, nb, offs, (const double*)hdxf, (const double*)rdiag,
                                       (const double*)rloc, (const double*)dep, h->G.as<double>(),
@@ -237,24 +271,26 @@ extern "C" int letkf_lpfgm_core_batched(letkf_handle_t* h, int32_t mode, double 
       HIP_TRY(launch_gram_csr<float>(K, cov_infl_gm, nb, offs, (const float*)hdxf, (const float*)rdiag,
                                      (const float*)rloc, (const float*)dep, h->G.as<double>(), h->bvec.as<double>(),
                                      h->nrow.as<int32_t>(), s));
-    HIP_TRY(launch_eig_fp64(K, nb, h->G.as<double>(), h->lam.as<double>(), h->V.as<double>(), h->status.as<int>(),
-                            0, nullptr, nullptr, 0, s));
-    lam = h->lam.as<double>();
-    Vp = h->V.as<double>();
+    Gp = h->G.as<double>();
     bv = h->bvec.as<double>();
     alpha = f64 ? (double)(K - 1) / cov_infl_gm : (double)((float)(K - 1) / (float)cov_infl_gm);
   }
+  const size_t lds = mode == 3 ? sizeof(double) * (size_t)K * K : 0;
+  if (lds > 65536) {
+    if (f64)
+      HIP_TRY(hipFuncSetAttribute((const void*)k_lpfgm<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    else
+      HIP_TRY(hipFuncSetAttribute((const void*)k_lpfgm<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   if (f64)
-    hipLaunchKernelGGL(k_lpfgm<double>, dim3(nb), dim3(256), 0, s, K, mode, nmonte, dp ? 1 : 0, alpha, offs,
+    hipLaunchKernelGGL(k_lpfgm<double>, dim3(nb), dim3(256), lds, s, K, mode, nmonte, dp ? 1 : 0, alpha, offs,
                        (const double*)hdxf, (const double*)rdiag, (const double*)rloc, (const double*)dep, d_rt, d_tt,
-                       lam, Vp, bv, (double*)wvec, (double*)wmat);
+                       Gp, bv, (double*)wvec, (double*)wmat);
   else
-    hipLaunchKernelGGL(k_lpfgm<float>, dim3(nb), dim3(256), 0, s, K, mode, nmonte, dp ? 1 : 0, alpha, offs,
+    hipLaunchKernelGGL(k_lpfgm<float>, dim3(nb), dim3(256), lds, s, K, mode, nmonte, dp ? 1 : 0, alpha, offs,
                        (const float*)hdxf, (const float*)rdiag, (const float*)rloc, (const float*)dep, d_rt, d_tt,
-                       lam, Vp, bv, (float*)wvec, (float*)wmat);
+                       Gp, bv, (float*)wvec, (float*)wmat);
   HIP_TRY(hipGetLastError());
-  int status = 0;
-  if (mode == 3) HIP_TRY(hipMemcpyAsync(&status, h->status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  return status_to_rc(status);
+  return LETKF_OK;
 }
