@@ -375,6 +375,32 @@ int letkf_lpfgm_core_batched(letkf_handle_t* h, int32_t mode,
                              const void* rdiag, const void* rloc,
                              const void* dep, void* wvec, void* wmat);
 
+/* Settings of the LPF / LPFGM analysis (iswitch_da, nmonte, RM, DP and
+ * cov_infl_gm of common_setting.f90:65, 79-89). */
+typedef struct letkf_lpfgm_cfg {
+  int32_t mode;        /* iswitch_da: 2 = LPF, 3 = LPFGM                     */
+  int32_t nmonte;      /* resamplings averaged per point (100), <= 255       */
+  int32_t rm_sr;       /* 0 = "MR" (multinomial), 1 = "SR" (systematic)      */
+  int32_t dp;          /* diagonal priority (.true.)                         */
+  double cov_infl_gm;  /* gm_core's multiplicative inflation (1.0)           */
+  const double* rand;  /* HOST nmonte x nmem uniforms in [0,1): the draws    */
+} letkf_lpfgm_cfg_t;
+
+/* da_main (common_da.f90:22-124) with iswitch_da = 2 or 3: at every point
+ * lpfgm_core (or lpfgm_core_noobs without local obs) replaces letkf_core,
+ * then Analysis_Ensemble, RTPP, RTPS and round_off as for the LETKF.
+ * DEVICE arrays as letkf_da_main_device; the handle's real_kind, sizes,
+ * localisation, inflation and round_off settings apply (its precision and
+ * form do not: the LPF core runs in FP64).  The draws are shared by every
+ * point, as the reference's clock-seeded draws are within one clock tick.
+ * nmem <= 128.  Runs on `stream` (NULL: the handle's) and returns when
+ * done. */
+int letkf_da_main_lpfgm_device(letkf_handle_t* h, const letkf_grid_t* grid,
+                               const letkf_obs_t* obs,
+                               const letkf_lpfgm_cfg_t* cfg, const void* fcst3d,
+                               const void* fcst2d, void* anal3d, void* anal2d,
+                               void* stream);
+
 /* ensemble_mesp (common.f90:505-596) on DEVICE arrays, as write_ens_mpi
  * calls it for the analysis and the forecast (common_io.f90:922-924):
  * v3d(nij1,nlev,nmem,nv3d), v2d(nij1,nmem,nv2d) in, v3dm/v3ds

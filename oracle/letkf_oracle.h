@@ -38,6 +38,11 @@ typedef struct oracle_params {
   double cov_infl_mul, alpha_rtpp, alpha_rtps;
   double sigma_obs, sigma_obsv, dist_zero, dist_zerov;
   double tmin, tmax, smin, smax;
+  /* iswitch_da (common_setting.f90:65): 0/1 LETKF, 2 LPF, 3 LPFGM; the
+   * LPF settings (79-89) and the resampling draws (nmonte x nmem) */
+  int32_t iswitch_da, nmonte, rm_sr, dp;
+  double cov_infl_gm;
+  const double* rand;
 } oracle_params_t;
 
 /* Geometry and observation tables of one analysis (borrowed pointers). */

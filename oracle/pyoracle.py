@@ -36,7 +36,8 @@ class OracleParams(C.Structure):
         "iv3d_s", "id_z_obs", "roff", "nthreads")] + [(n, C.c_double) for n in (
         "pi", "re", "cov_infl_mul", "alpha_rtpp", "alpha_rtps", "sigma_obs",
         "sigma_obsv", "dist_zero", "dist_zerov", "tmin", "tmax", "smin",
-        "smax")]
+        "smax")] + [(n, C.c_int32) for n in ("iswitch_da", "nmonte", "rm_sr", "dp")] + [
+        ("cov_infl_gm", C.c_double), ("rand", C.POINTER(C.c_double))]
 
 
 def _tables_struct(ct):
@@ -338,10 +339,22 @@ def obs_local(pr: Problem, i: int, k: int, dtype=np.float64):
     return idx[:n], hd[:n], rd[:n], rl[:n], dp[:n]
 
 
-def da_main(pr: Problem, dtype=np.float64, nthreads: int = 0):
-    """Full analysis.  Returns (rc, anal3d, anal2d, nobsl(nij1,nlev))."""
+def da_main(pr: Problem, dtype=np.float64, nthreads: int = 0, lpfgm=None):
+    """Full analysis.  Returns (rc, anal3d, anal2d, nobsl(nij1,nlev)).
+    lpfgm: None (LETKF, iswitch_da 1) or a dict(mode=2|3, rand=(nmonte,
+    nmem) draws, cov_infl_gm=1.0, rm="MR"|"SR", dp=True) selecting the
+    LPF / LPFGM core (iswitch_da 2 / 3)."""
     ct = _ct(dtype)
     op = make_params(pr, nthreads)
+    keep = None
+    if lpfgm is not None:
+        keep = np.ascontiguousarray(lpfgm["rand"], np.float64)
+        op.iswitch_da = int(lpfgm.get("mode", 3))
+        op.nmonte = keep.shape[0]
+        op.rm_sr = 1 if lpfgm.get("rm", "MR") == "SR" else 0
+        op.dp = 1 if lpfgm.get("dp", True) else 0
+        op.cov_infl_gm = float(lpfgm.get("cov_infl_gm", 1.0))
+        op.rand = keep.ctypes.data_as(C.POINTER(C.c_double))
     t = make_tables(pr, dtype)
     f3 = _f(pr.fcst3d, dtype)
     f2 = _f(pr.fcst2d, dtype)

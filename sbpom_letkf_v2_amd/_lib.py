@@ -26,7 +26,14 @@ EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_set_params", "letkf_get_stats", "letkf_last_error", "letkf_trans_xtoy_device",
            "letkf_setobs_default", "letkf_set_obs_device", "letkf_da_main_plan", "letkf_da_main_enqueue",
            "letkf_da_main_status", "letkf_autotune", "letkf_set_tuning", "letkf_get_tuning",
-           "letkf_ensemble_mesp_device", "letkf_grid_pack_device", "letkf_lpfgm_core_batched")
+           "letkf_ensemble_mesp_device", "letkf_grid_pack_device", "letkf_lpfgm_core_batched",
+           "letkf_da_main_lpfgm_device")
+
+
+class LpfgmCfg(C.Structure):
+    """letkf_lpfgm_cfg_t: iswitch_da, nmonte, RM, DP, cov_infl_gm, draws."""
+    _fields_ = [(n, C.c_int32) for n in ("mode", "nmonte", "rm_sr", "dp")] + [
+        ("cov_infl_gm", C.c_double), ("rand", C.POINTER(C.c_double))]
 
 
 class Params(C.Structure):
@@ -133,6 +140,8 @@ def lib():
         L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
         i32 = C.c_int32
         L.letkf_ensemble_mesp_device.argtypes = [i32] * 6 + [C.c_double] + [vp] * 7
+        L.letkf_da_main_lpfgm_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), C.POINTER(LpfgmCfg),
+                                                 vp, vp, vp, vp, vp]
         L.letkf_lpfgm_core_batched.argtypes = ([vp, i32, C.c_double, i32, i32, i32, C.POINTER(C.c_double), i32]
                                                + [vp] * 7)
         L.letkf_grid_pack_device.argtypes = ([i32] * 9 + [vp, C.POINTER(C.c_int64), vp, C.POINTER(C.c_int64)]

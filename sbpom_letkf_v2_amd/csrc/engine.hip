@@ -753,6 +753,90 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   return status_to_rc(status);
 }
 
+// da_main with the LPF / LPFGM core (iswitch_da 2/3): search count, point
+// scans, then per chunk the search fill, for LPFGM the FP64 Gram with alpha
+// = (K-1)/cov_infl_gm, and the per-point core + ensemble update kernel.
+template <typename T>
+static int run_lpfgm(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const letkf_lpfgm_cfg_t* cfg,
+                     const T* f3, const T* f2, T* a3, T* a2, hipStream_t s) {
+  const letkf_params_t& p = h->prm;
+  const int K = p.nmem;
+  KParams kp = make_kparams(p, o->nobs);
+  const long long P = kp.P;
+  if (o->nobs == 0) {  // analysis = forecast (common_da.f90:57-61)
+    if (p.nv3d) HIP_TRY(hipMemcpyAsync(a3, f3, sizeof(T) * P * K * p.nv3d, hipMemcpyDeviceToDevice, s));
+    if (p.nv2d)
+      HIP_TRY(hipMemcpyAsync(a2, f2, sizeof(T) * (size_t)p.nij1 * K * p.nv2d, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return LETKF_OK;
+  }
+  LpfPointArgs la{cfg->mode, cfg->nmonte, cfg->dp ? 1 : 0, 0.0, nullptr, nullptr};
+  HIP_TRY(lpf_prepare(K, cfg->nmonte, cfg->rm_sr, sizeof(T) == 8, cfg->rand, h->bin_tmp, s, &la.rtab, &la.ttab));
+  la.alpha = (double)((T)(K - 1) / (T)cfg->cov_infl_gm);
+  KParams kg = kp;
+  kg.rho = cfg->cov_infl_gm;  // the Gram's diagonal (K-1)/rho
+  HIP_TRY(h->otrig.ensure(sizeof(double2) * (size_t)(o->nobs + 1)));
+  HIP_TRY(launch_obs_trig<T>(o->nobs, p.pi, (const T*)o->lat, h->otrig.as<double2>(), s));
+  const Tables<T> tb = make_tables<T>(g, o, h->otrig.as<double2>());
+  HIP_TRY(h->nloc.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->loff.ensure(sizeof(int64_t) * (P + 1)));
+  HIP_TRY(h->actidx.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->act_p.ensure(sizeof(int32_t) * (P + 1)));
+  HIP_TRY(h->scan_tmp.ensure(scan_tmp_bytes(P)));
+  HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
+  HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
+  HIP_TRY(launch_search_count<T>(kp, tb, h->nloc.as<int32_t>(), s));
+  HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
+                             h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
+  // chunks with a K x K slot per active point (~2 GiB)
+  long long PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));
+  PC = std::max(1024LL, std::min(PC, P));
+  const long long nch = (P + PC - 1) / PC;
+  std::vector<long long> bpts(nch + 1);
+  for (long long c = 0; c <= nch; ++c) bpts[c] = std::min(P, c * PC);
+  HIP_TRY(h->bounds.ensure(sizeof(long long) * (3 * (nch + 1) + 5)));
+  long long* d_pts = h->bounds.as<long long>();
+  long long* d_out = d_pts + (nch + 1);
+  HIP_TRY(hipMemcpyAsync(d_pts, bpts.data(), sizeof(long long) * (nch + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_gather_bounds, dim3((unsigned)((nch + 1 + 255) / 256)), dim3(256), 0, s, (int)(nch + 1),
+                     d_pts, h->loff.as<int64_t>(), h->actidx.as<int32_t>(), d_out);
+  HIP_TRY(hipGetLastError());
+  std::vector<long long> bv(2 * (nch + 1));
+  HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * 2 * (nch + 1), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  long long maxL = 0, maxA = 0;
+  for (long long c = 0; c < nch; ++c) {
+    maxL = std::max(maxL, bv[2 * (c + 1)] - bv[2 * c]);
+    maxA = std::max(maxA, bv[2 * (c + 1) + 1] - bv[2 * c + 1]);
+  }
+  HIP_TRY(h->lobs.ensure(sizeof(int32_t) * (maxL + 1)));
+  HIP_TRY(h->lrloc.ensure(sizeof(double) * (maxL + 1)));
+  if (cfg->mode == 3) {
+    HIP_TRY(h->G.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
+    HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+  }
+  for (long long c = 0; c < nch; ++c) {
+    const long long p0 = bpts[c], p1 = bpts[c + 1];
+    const long long lbase = bv[2 * c];
+    const int a0 = (int)bv[2 * c + 1];
+    const int na = (int)(bv[2 * (c + 1) + 1] - a0);
+    if (na > 0) {
+      HIP_TRY(launch_search_fill<T>(kp, tb, p0, p1, h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
+                                    h->lrloc.as<double>(), s));
+      if (cfg->mode == 3)
+        HIP_TRY(launch_gram_fp64<T>(kg, tb, h->yT.as<double>(), h->act_p.as<int32_t>(), h->loff.as<int64_t>(),
+                                    lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na, h->G.as<double>(),
+                                    h->bvec.as<double>(), 0, s));
+    }
+    HIP_TRY(launch_lpfgm_points<T>(kp, tb, f3, f2, a3, a2, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), lbase,
+                                   h->lobs.as<int32_t>(), h->lrloc.as<double>(), h->yT.as<double>(),
+                                   h->actidx.as<int32_t>(), a0, cfg->mode == 3 ? h->G.as<double>() : nullptr,
+                                   cfg->mode == 3 ? h->bvec.as<double>() : nullptr, p0, p1, la, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return LETKF_OK;
+}
+
 static int check_call(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
                       const void* f2, void* a3, void* a2) {
   if (!h) return set_err(LETKF_ERR_ARG, "handle is NULL");
@@ -779,6 +863,24 @@ extern "C" int letkf_da_main_device(letkf_handle_t* h, const letkf_grid_t* g, co
   if (h->prm.real_kind == 8)
     return run_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s);
   return run_device<float>(h, g, o, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s);
+}
+
+extern "C" int letkf_da_main_lpfgm_device(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o,
+                                          const letkf_lpfgm_cfg_t* cfg, const void* f3, const void* f2, void* a3,
+                                          void* a2, void* stream) {
+  int rc = check_call(h, g, o, f3, f2, a3, a2);
+  if (rc) return rc;
+  if (!cfg) return set_err(LETKF_ERR_ARG, "lpfgm settings are NULL");
+  if (cfg->mode != 2 && cfg->mode != 3) return set_err(LETKF_ERR_ARG, "lpfgm mode must be 2 or 3 (got %d)", cfg->mode);
+  if (cfg->nmonte < 1 || cfg->nmonte > 255 || !cfg->rand)
+    return set_err(LETKF_ERR_ARG, "lpfgm needs 1 <= nmonte <= 255 and the draws");
+  if (h->prm.nmem > 128 || h->prm.nmem < 2) return set_err(LETKF_ERR_ARG, "lpfgm analysis needs 2 <= nmem <= 128");
+  if (cfg->mode == 3 && !(cfg->cov_infl_gm > 0.0)) return set_err(LETKF_ERR_ARG, "cov_infl_gm must be > 0");
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  if (h->prm.real_kind == 8)
+    return run_lpfgm<double>(h, g, o, cfg, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s);
+  return run_lpfgm<float>(h, g, o, cfg, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s);
 }
 
 // In-project autotuner of the register eigensolve's vector path: on the

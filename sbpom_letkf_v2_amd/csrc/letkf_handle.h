@@ -113,3 +113,10 @@ struct letkf_handle {
 
 // Kernel status bits -> return code (with the error text set).
 int status_to_rc(int status);
+
+namespace letkf {
+// Resampling draws (sorted for MR, the SR formula) and the running sums of
+// 1/nmonte for the LPF kernels, uploaded into buf (lpfgm.hip).
+hipError_t lpf_prepare(int K, int nmonte, int rm_sr, bool f64, const double* rand, DevBuf& buf, hipStream_t s,
+                       const double** d_rt, const double** d_tt);
+}  // namespace letkf

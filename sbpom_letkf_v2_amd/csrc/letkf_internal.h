@@ -108,6 +108,23 @@ hipError_t launch_form_weights(int K, double rho, int nb, const int32_t* nrow,
                                const double* bvec, T* wvec, T* wmat,
                                hipStream_t s);
 
+// LPF / LPFGM inside da_main (iswitch_da 2/3): settings, draws and the
+// running-sum table on the device, and the per-point kernel over points
+// [p0, p1) of a chunk (G/bvec: the chunk's K x K Gram slots, mode 3 only;
+// the kernel overwrites a point's slot with its W).
+struct LpfPointArgs {
+  int mode, nmonte, dp;
+  double alpha;  // (K-1)/cov_infl_gm in the real kind
+  const double* rtab;
+  const double* ttab;
+};
+template <typename T>
+hipError_t launch_lpfgm_points(const KParams& kp, const Tables<T>& tb, const T* f3, const T* f2, T* a3, T* a2,
+                               const int32_t* nloc, const int64_t* loff, long long lbase, const int32_t* lobs,
+                               const double* lrloc, const double* yT, const int32_t* actidx, int a0, double* G,
+                               const double* bvec, long long p0, long long p1, const LpfPointArgs& la,
+                               hipStream_t s);
+
 // AOEI and obs binning pre-passes.
 template <typename T>
 hipError_t launch_aoei(int nobs, T* err, const T* dep, const T* sprd,

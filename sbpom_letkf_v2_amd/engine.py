@@ -153,6 +153,28 @@ class Engine:
                                               ptr(pr.fcst2d), ptr(anal3d), ptr(anal2d), s))
         return anal3d, anal2d
 
+    def da_main_lpfgm_device(self, pr, rand, mode=3, cov_infl_gm=1.0, rm="MR", dp=True, anal3d=None,
+                             anal2d=None, stream=None):
+        """letkf_da_main_lpfgm_device: da_main with iswitch_da = mode (2 LPF,
+        3 LPFGM; common_da.f90:74-84, common_lpfgm.f90).  rand (nmonte, nmem)
+        uniforms in [0, 1) are the draws of the resamplings."""
+        import numpy as np
+        import torch
+        g, o = self._device_structs(pr)
+        r = np.ascontiguousarray(rand, np.float64)
+        if r.ndim != 2 or r.shape[1] != self.params.nmem:
+            raise ValueError("rand must be (nmonte, nmem)")
+        cfg = _lib.LpfgmCfg(int(mode), r.shape[0], 1 if rm == "SR" else 0, 1 if dp else 0, float(cov_infl_gm),
+                            r.ctypes.data_as(C.POINTER(C.c_double)))
+        if anal3d is None:
+            anal3d = torch.empty_like(pr.fcst3d)
+        if anal2d is None:
+            anal2d = torch.empty_like(pr.fcst2d)
+        s = C.c_void_p(stream) if stream else None
+        check(_lib.lib().letkf_da_main_lpfgm_device(self.h, C.byref(g), C.byref(o), C.byref(cfg), _ptr(pr.fcst3d),
+                                                    _ptr(pr.fcst2d), _ptr(anal3d), _ptr(anal2d), s))
+        return anal3d, anal2d
+
     BUCKETS = (16, 32, 48, 64, 128)
 
     def autotune(self, pr, anal3d=None, anal2d=None, stream=None):

@@ -115,6 +115,8 @@ def test_argument_errors_are_status_codes():
     assert L.letkf_trans_xtoy_device(C.byref(xg), 1, *([None] * 6), 2, None, None, None, None, None,
                                      None) == -1
     assert L.letkf_lpfgm_core_batched(None, 3, 1.0, 100, 0, 1, None, 0, *([None] * 7)) == -1
+    lc = _lib.LpfgmCfg()
+    assert L.letkf_da_main_lpfgm_device(None, C.byref(g), C.byref(o), C.byref(lc), *([None] * 5)) == -1
     # the output-step entries check kinds and sizes before any HIP call
     assert L.letkf_ensemble_mesp_device(3, 10, 2, 4, 1, 1, 999.0, *([None] * 7)) == -1
     assert L.letkf_ensemble_mesp_device(8, 10, 2, 0, 1, 1, 999.0, *([None] * 7)) == -1
