@@ -43,7 +43,8 @@ def _case(real_kind, mode, nmem=16, rm="MR", dp=True, nobs=500, seed=3, **kw):
 @pytest.mark.parametrize("rm,dp", [("MR", True), ("SR", True), ("MR", False)])
 def test_lpf_da_main_bit_exact(rm, dp):
     _say(f"case lpf da_main FP64 {rm} dp={dp}")
-    g3, g2, a3o, a2o, nl = _case(8, 2, rm=rm, dp=dp)
+    # tight localisation: points with and without local obs (W = I)
+    g3, g2, a3o, a2o, nl = _case(8, 2, rm=rm, dp=dp, nobs=60, sigma_h_cells=1.0, sigma_v=0.5)
     assert (nl > 0).sum() > 0 and (nl == 0).sum() > 0
     np.testing.assert_array_equal(g3, a3o)
     np.testing.assert_array_equal(g2, a2o)
