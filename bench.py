@@ -68,6 +68,9 @@ def parse():
                     help="time the LPFGM core (letkf_lpfgm_core_batched) on local problems of nmem 100 and 84 "
                          "obs, with an oracle check and CPU baseline on a sample")
     ap.add_argument("--lpfgm-problems", type=int, default=50000)
+    ap.add_argument("--lpfgm-da", action="store_true",
+                    help="time da_main with the LPFGM core (iswitch_da 3) on the workload, with an oracle "
+                         "check and CPU baseline on sampled points")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise kernel that reads and writes 1 GiB (three launches), the "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -435,6 +438,59 @@ def run_lpfgm(args):
     print(json.dumps(out))
 
 
+def run_lpfgm_da(args):
+    """da_main with the LPFGM core (iswitch_da 3; letkf_da_main_lpfgm_device)
+    on a workload (default c2), 100 resamplings per point, MR with diagonal
+    priority, FP64; checked against the FP64 oracle with the same switch on a
+    sample of points, which is also the CPU baseline."""
+    import numpy as np
+    import torch
+    import pyoracle as po
+    from sbpom_letkf_v2_amd import engine, synth
+    wl = args.workload
+    log(f"generating workload {wl} on the GPU")
+    prd = synth.config(wl, device="cuda", real_kind=8, progress=log)
+    meta = prd.meta
+    K, nv3, nv2 = meta["nmem"], meta["nv3d"], meta["nv2d"]
+    P = meta["nlon"] * meta["nlat"] * meta["nlev"]
+    draws = np.random.default_rng(11).uniform(size=(100, K))
+    lp = dict(mode=3, rand=draws, cov_infl_gm=1.0, rm="MR", dp=True)
+    eng = engine.Engine(engine.make_params(prd))
+    a3 = torch.empty_like(prd.fcst3d)
+    a2 = torch.empty_like(prd.fcst2d)
+    call = lambda: eng.da_main_lpfgm_device(prd, draws, mode=3, anal3d=a3, anal2d=a2)
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    eng.close()
+    log(f"  LPFGM da_main {ms:.1f} ms per analysis, {P / (ms * 1e-3):.3e} grid points/s")
+    cpu = parity = None
+    if not args.no_cpu:
+        pn_tabs = synth.Problem(
+            params=dict(prd.params), **{k: getattr(prd, k) for k in synth.Problem.ARRAYS
+                                        if k not in ("fcst3d", "fcst2d")},
+            fcst3d=torch.zeros((nv3, K, 1, 1), dtype=torch.float64),
+            fcst2d=torch.zeros((max(nv2, 1), K, 1), dtype=torch.float64)).numpy()
+        opr = po.Problem(params=dict(pn_tabs.params), **{k: getattr(pn_tabs, k) for k in synth.Problem.ARRAYS})
+        cpu, parity, npt = cpu_baseline(opr, prd, a3, a2, args.cpu_seconds, K, nv3, nv2, lpfgm=lp)
+        log(f"  cpu {cpu['value']:.0f} pts/s on {cpu['cores']} cores; GPU vs oracle max rel err {parity:.2e} "
+            f"over {npt} points")
+        if parity > 1e-9:
+            raise SystemExit("LPFGM analysis differs from the oracle")
+    out = {"metric": "LPFGM analysed grid points per second", "value": P / (ms * 1e-3), "unit": "grid points/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{wl}: {meta['nlon']}x{meta['nlat']}x{meta['nlev']} K={K}, iswitch_da 3 "
+                                  "(LPFGM), nmonte 100, MR + DP"},
+           "parity_max_rel_err_vs_fp64_oracle_sample": parity, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
 def run_c4(args):
     """Workload c4: eight independent analysis cycles (grids 128^2 .. 512^2 x
     40, nmem 32/64/100/128 round-robin, 0.5 .. 4 random obs per column),
@@ -663,7 +719,7 @@ def graph_replay(prd, precision, form, steps, a3_eager, a2_eager, paths=None):
     return {"ms_per_replay": ms, "plan_ms": plan_ms, "bit_exact_vs_eager": same}
 
 
-def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2):
+def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2, lpfgm=None):
     """FP64 oracle (the C restatement, OpenMP) on a uniform random sample of
     points of the same workload; also returns the GPU-vs-oracle error at the
     sampled points (full-size parity)."""
@@ -693,12 +749,12 @@ def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2):
     # calibrate on a small sample, then size the timed sample to ~target_s
     pi, pk, x3, x2, _, _ = sample(64 * cores)
     t0 = time.perf_counter()
-    po.analyse_points(pr_host_tables, pi, pk, x3, x2, nthreads=cores)
+    po.analyse_points(pr_host_tables, pi, pk, x3, x2, nthreads=cores, lpfgm=lpfgm)
     dt = time.perf_counter() - t0
     npt = int(min(P, max(64 * cores, 64 * cores * target_s / max(dt, 1e-3))))
     pi, pk, x3, x2, g3, g2 = sample(npt)
     t0 = time.perf_counter()
-    rc, xa3, xa2, nl = po.analyse_points(pr_host_tables, pi, pk, x3, x2, nthreads=cores)
+    rc, xa3, xa2, nl = po.analyse_points(pr_host_tables, pi, pk, x3, x2, nthreads=cores, lpfgm=lpfgm)
     dt = time.perf_counter() - t0
     assert rc == 0, rc
     err3 = float(np.abs(g3 - xa3).max() / np.abs(xa3).max())
@@ -754,6 +810,8 @@ def main():
         return run_output(args)
     if args.lpfgm:
         return run_lpfgm(args)
+    if args.lpfgm_da:
+        return run_lpfgm_da(args)
     if args.xtoy:
         run_xtoy(args)
         return

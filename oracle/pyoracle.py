@@ -368,11 +368,21 @@ def da_main(pr: Problem, dtype=np.float64, nthreads: int = 0, lpfgm=None):
 
 
 def analyse_points(pr_tables: Problem, pt_i, pt_k, xf3, xf2, dtype=np.float64,
-                   nthreads: int = 0):
+                   nthreads: int = 0, lpfgm=None):
     """Analysis of an explicit list of points.  xf3: (npt, nv3d, nmem) packed
-    as the (nmem, nv3d) column-major slice per point; xf2 likewise."""
+    as the (nmem, nv3d) column-major slice per point; xf2 likewise.  lpfgm
+    as for da_main."""
     ct = _ct(dtype)
     op = make_params(pr_tables, nthreads)
+    keep = None
+    if lpfgm is not None:
+        keep = np.ascontiguousarray(lpfgm["rand"], np.float64)
+        op.iswitch_da = int(lpfgm.get("mode", 3))
+        op.nmonte = keep.shape[0]
+        op.rm_sr = 1 if lpfgm.get("rm", "MR") == "SR" else 0
+        op.dp = 1 if lpfgm.get("dp", True) else 0
+        op.cov_infl_gm = float(lpfgm.get("cov_infl_gm", 1.0))
+        op.rand = keep.ctypes.data_as(C.POINTER(C.c_double))
     t = make_tables(pr_tables, dtype)
     pt_i = np.ascontiguousarray(pt_i, np.int32)
     pt_k = np.ascontiguousarray(pt_k, np.int32)
