@@ -51,15 +51,27 @@ __device__ void lpf_counts(LpfShared& sh, int K, int n, int nmonte, int dp, cons
   const int t = threadIdx.x;
   for (int q = t; q < K * K / 4 + 1 && q < kLpfMaxK * kLpfMaxK / 4; q += 256) sh.cnt[q] = 0u;
   for (int q = t; q <= nmonte; q += 256) sh.tt[q] = ttab[q];
-  if (t < K) {  // log-likelihood per particle, rows in order
+  if (t < K) {  // log-likelihood per particle, rows in order, 8 rows of loads in flight
     double s = 0.0;
-    for (int r = 0; r < n; ++r) {
-      const auto dx = rows.dx(r, t);
-      double x = 0.5 * (double)dx;
-      x = x * (double)dx;
-      x = x * rows.rloc(r);
-      x = x / rows.rdiag(r);
-      s = s - x;
+    for (int r0 = 0; r0 < n; r0 += 8) {
+      decltype(rows.dx(0, 0)) dx[8];
+      double rl[8], rd[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = r0 + u < n ? r0 + u : r0;
+        dx[u] = rows.dx(r, t);
+        rl[u] = rows.rloc(r);
+        rd[u] = rows.rdiag(r);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (r0 + u >= n) break;
+        double x = 0.5 * (double)dx[u];
+        x = x * (double)dx[u];
+        x = x * rl[u];
+        x = x / rd[u];
+        s = s - x;
+      }
     }
     sh.ll[t] = s;
   }
@@ -166,12 +178,19 @@ __device__ void gm_times_lpf(LpfShared& sh, double* cm, int K, double alpha, con
   // (Linv(i,j) at cm[j*K + i], i > j); its diagonal in il
   if (t < K) sh.il[t] = 1.0 / cm[t * K + t];
   __syncthreads();
-  if (t < K) {
+  if (t < K) {  // four partial sums per dot product: LDS latency, not the chain, sets the pace
     const int j = t;
     for (int i = j + 1; i < K; ++i) {
-      double acc = cm[i * K + j] * sh.il[j];
-      for (int k = j + 1; k < i; ++k) acc = fma(cm[i * K + k], cm[j * K + k], acc);
-      cm[j * K + i] = -acc * sh.il[i];
+      double a0 = cm[i * K + j] * sh.il[j], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int k = j + 1;
+      for (; k + 4 <= i; k += 4) {
+        a0 = fma(cm[i * K + k], cm[j * K + k], a0);
+        a1 = fma(cm[i * K + k + 1], cm[j * K + k + 1], a1);
+        a2 = fma(cm[i * K + k + 2], cm[j * K + k + 2], a2);
+        a3 = fma(cm[i * K + k + 3], cm[j * K + k + 3], a3);
+      }
+      for (; k < i; ++k) a0 = fma(cm[i * K + k], cm[j * K + k], a0);
+      cm[j * K + i] = -((a0 + a1) + (a2 + a3)) * sh.il[i];
     }
   }
   __syncthreads();
@@ -188,12 +207,19 @@ __device__ void gm_times_lpf(LpfShared& sh, double* cm, int K, double alpha, con
     sh.wb[t] = acc;
   }
   __syncthreads();
-  for (int q = t; q < K * K; q += 256) {
+  for (int q = t; q < K * K; q += 256) {  // Pa(i,j) = sum_k Linv(k,i) Linv(k,j), k >= max(i,j)
     const int i = q / K, j = q - i * K;
     const int k0 = i > j ? i : j;
-    double acc = 0.0;
-    for (int k = k0; k < K; ++k) acc = fma(linv(k, i), linv(k, j), acc);
-    Wb[q] = (WT)(alpha * acc + sh.wb[i]);
+    double a0 = linv(k0, i) * linv(k0, j), a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = k0 + 1;
+    for (; k + 4 <= K; k += 4) {
+      a0 = fma(cm[i * K + k], cm[j * K + k], a0);
+      a1 = fma(cm[i * K + k + 1], cm[j * K + k + 1], a1);
+      a2 = fma(cm[i * K + k + 2], cm[j * K + k + 2], a2);
+      a3 = fma(cm[i * K + k + 3], cm[j * K + k + 3], a3);
+    }
+    for (; k < K; ++k) a0 = fma(cm[i * K + k], cm[j * K + k], a0);
+    Wb[q] = (WT)(alpha * ((a0 + a1) + (a2 + a3)) + sh.wb[i]);
   }
   __syncthreads();
   for (int i0 = 0; i0 < K; i0 += kLpfRows) {  // W = W_GM W_LPF, rows staged in cm
@@ -202,9 +228,16 @@ __device__ void gm_times_lpf(LpfShared& sh, double* cm, int K, double alpha, con
     __syncthreads();
     for (int q = t; q < nr * K; q += 256) {
       const int i = q / K, k = q - i * K;
-      double acc = 0.0;
-      for (int j = 0; j < K; ++j) acc = fma(cm[i * K + j], lpf_w(sh, K, j, k), acc);
-      Wb[(long long)(i0 + i) * K + k] = (WT)acc;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int j = 0;
+      for (; j + 4 <= K; j += 4) {
+        a0 = fma(cm[i * K + j], lpf_w(sh, K, j, k), a0);
+        a1 = fma(cm[i * K + j + 1], lpf_w(sh, K, j + 1, k), a1);
+        a2 = fma(cm[i * K + j + 2], lpf_w(sh, K, j + 2, k), a2);
+        a3 = fma(cm[i * K + j + 3], lpf_w(sh, K, j + 3, k), a3);
+      }
+      for (; j < K; ++j) a0 = fma(cm[i * K + j], lpf_w(sh, K, j, k), a0);
+      Wb[(long long)(i0 + i) * K + k] = (WT)((a0 + a1) + (a2 + a3));
     }
     __syncthreads();
   }
