@@ -395,6 +395,12 @@ typedef struct letkf_lpfgm_cfg {
  * point, as the reference's clock-seeded draws are within one clock tick.
  * nmem <= 128.  Runs on `stream` (NULL: the handle's) and returns when
  * done. */
+/* The same on HOST arrays (as letkf_da_main): what the Fortran driver calls
+ * in place of da_main when iswitch_da is 2 or 3. */
+int letkf_da_main_lpfgm(letkf_handle_t* h, const letkf_grid_t* grid,
+                        const letkf_obs_t* obs, const letkf_lpfgm_cfg_t* cfg,
+                        const void* fcst3d, const void* fcst2d, void* anal3d,
+                        void* anal2d);
 int letkf_da_main_lpfgm_device(letkf_handle_t* h, const letkf_grid_t* grid,
                                const letkf_obs_t* obs,
                                const letkf_lpfgm_cfg_t* cfg, const void* fcst3d,

@@ -27,7 +27,7 @@ EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_setobs_default", "letkf_set_obs_device", "letkf_da_main_plan", "letkf_da_main_enqueue",
            "letkf_da_main_status", "letkf_autotune", "letkf_set_tuning", "letkf_get_tuning",
            "letkf_ensemble_mesp_device", "letkf_grid_pack_device", "letkf_lpfgm_core_batched",
-           "letkf_da_main_lpfgm_device")
+           "letkf_da_main_lpfgm_device", "letkf_da_main_lpfgm")
 
 
 class LpfgmCfg(C.Structure):
@@ -142,6 +142,7 @@ def lib():
         L.letkf_ensemble_mesp_device.argtypes = [i32] * 6 + [C.c_double] + [vp] * 7
         L.letkf_da_main_lpfgm_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), C.POINTER(LpfgmCfg),
                                                  vp, vp, vp, vp, vp]
+        L.letkf_da_main_lpfgm.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), C.POINTER(LpfgmCfg), vp, vp, vp, vp]
         L.letkf_lpfgm_core_batched.argtypes = ([vp, i32, C.c_double, i32, i32, i32, C.POINTER(C.c_double), i32]
                                                + [vp] * 7)
         L.letkf_grid_pack_device.argtypes = ([i32] * 9 + [vp, C.POINTER(C.c_int64), vp, C.POINTER(C.c_int64)]

@@ -1052,8 +1052,11 @@ extern "C" int letkf_set_params(letkf_handle_t* h, const letkf_params_t* p) {
   return LETKF_OK;
 }
 
-extern "C" int letkf_da_main(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
-                             const void* f2, void* a3, void* a2) {
+// Host arrays: staged into the handle's device copies, analysed by the
+// device entry (the LETKF, or the LPF / LPFGM core when cfg is given), the
+// analysis copied back.
+static int da_main_host(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o,
+                        const letkf_lpfgm_cfg_t* cfg, const void* f3, const void* f2, void* a3, void* a2) {
   int rc = check_call(h, g, o, f3, f2, a3, a2);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(h->device));
@@ -1103,10 +1106,23 @@ extern "C" int letkf_da_main(letkf_handle_t* h, const letkf_grid_t* g, const let
   dobs.hdxf = put(o->hdxf, R * nobs * K);
   if (n3) HIP_TRY(hipMemcpyAsync(h->h_f3.ptr, f3, R * n3, hipMemcpyHostToDevice, s));
   if (n2) HIP_TRY(hipMemcpyAsync(h->h_f2.ptr, f2, R * n2, hipMemcpyHostToDevice, s));
-  rc = letkf_da_main_device(h, &dg, &dobs, h->h_f3.ptr, h->h_f2.ptr, h->h_a3.ptr, h->h_a2.ptr, s);
+  rc = cfg ? letkf_da_main_lpfgm_device(h, &dg, &dobs, cfg, h->h_f3.ptr, h->h_f2.ptr, h->h_a3.ptr, h->h_a2.ptr, s)
+           : letkf_da_main_device(h, &dg, &dobs, h->h_f3.ptr, h->h_f2.ptr, h->h_a3.ptr, h->h_a2.ptr, s);
   if (rc) return rc;
   if (n3) HIP_TRY(hipMemcpyAsync(a3, h->h_a3.ptr, R * n3, hipMemcpyDeviceToHost, s));
   if (n2) HIP_TRY(hipMemcpyAsync(a2, h->h_a2.ptr, R * n2, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return LETKF_OK;
+}
+
+extern "C" int letkf_da_main(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
+                             const void* f2, void* a3, void* a2) {
+  return da_main_host(h, g, o, nullptr, f3, f2, a3, a2);
+}
+
+extern "C" int letkf_da_main_lpfgm(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o,
+                                   const letkf_lpfgm_cfg_t* cfg, const void* f3, const void* f2, void* a3,
+                                   void* a2) {
+  if (!cfg) return set_err(LETKF_ERR_ARG, "lpfgm settings are NULL");
+  return da_main_host(h, g, o, cfg, f3, f2, a3, a2);
 }

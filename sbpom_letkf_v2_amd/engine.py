@@ -94,7 +94,9 @@ class Engine:
         return out
 
     # -------------------------------------------------------- host arrays
-    def da_main_host(self, pr):
+    def da_main_host(self, pr, lpfgm=None):
+        """letkf_da_main on host arrays (Fortran layouts), or with lpfgm =
+        dict(mode, rand, cov_infl_gm, rm, dp) letkf_da_main_lpfgm."""
         dt = np.float32 if self.params.real_kind == 4 else np.float64
         keep = []
 
@@ -113,7 +115,15 @@ class Engine:
         a3 = np.zeros_like(f3, order="F")
         a2 = np.zeros_like(f2, order="F")
         vp = lambda a: a.ctypes.data_as(C.c_void_p)
-        check(_lib.lib().letkf_da_main(self.h, C.byref(g), C.byref(o), vp(f3), vp(f2), vp(a3), vp(a2)))
+        if lpfgm is None:
+            check(_lib.lib().letkf_da_main(self.h, C.byref(g), C.byref(o), vp(f3), vp(f2), vp(a3), vp(a2)))
+            return a3, a2
+        r = np.ascontiguousarray(lpfgm["rand"], np.float64)
+        cfg = _lib.LpfgmCfg(int(lpfgm.get("mode", 3)), r.shape[0], 1 if lpfgm.get("rm", "MR") == "SR" else 0,
+                            1 if lpfgm.get("dp", True) else 0, float(lpfgm.get("cov_infl_gm", 1.0)),
+                            r.ctypes.data_as(C.POINTER(C.c_double)))
+        check(_lib.lib().letkf_da_main_lpfgm(self.h, C.byref(g), C.byref(o), C.byref(cfg), vp(f3), vp(f2), vp(a3),
+                                             vp(a2)))
         return a3, a2
 
     def set_params(self, params: _lib.Params):

@@ -15,8 +15,8 @@ MODULE letkf_engine_mod
   USE iso_c_binding
   IMPLICIT NONE
   PRIVATE
-  PUBLIC :: letkf_params_t, letkf_grid_t, letkf_obs_t
-  PUBLIC :: letkf_params_default, letkf_create, letkf_destroy, letkf_da_main
+  PUBLIC :: letkf_params_t, letkf_grid_t, letkf_obs_t, letkf_lpfgm_cfg_t
+  PUBLIC :: letkf_params_default, letkf_create, letkf_destroy, letkf_da_main, letkf_da_main_lpfgm
   PUBLIC :: letkf_last_error_f
   PUBLIC :: LETKF_PREC_FP64, LETKF_PREC_FP32, LETKF_PREC_BF16X3, LETKF_PREC_FP32_PANEL, LETKF_PREC_FP32X3, &
             LETKF_PREC_FP32_DUAL64
@@ -44,6 +44,14 @@ MODULE letkf_engine_mod
      TYPE(c_ptr) :: nobsgrd, elm, lon, lat, lev, err, dep, hdxf
   END TYPE letkf_obs_t
 
+  ! iswitch_da, nmonte, RM, DP, cov_infl_gm (common_setting.f90:65,79-89)
+  ! and the nmonte x nmem resampling draws
+  TYPE, BIND(C) :: letkf_lpfgm_cfg_t
+     INTEGER(c_int32_t) :: mode, nmonte, rm_sr, dp
+     REAL(c_double) :: cov_infl_gm
+     TYPE(c_ptr) :: rand
+  END TYPE letkf_lpfgm_cfg_t
+
   INTERFACE
      INTEGER(c_int) FUNCTION letkf_params_default(p, real_kind) BIND(C, name="letkf_params_default")
        IMPORT :: c_int, c_int32_t, letkf_params_t
@@ -70,6 +78,16 @@ MODULE letkf_engine_mod
        TYPE(letkf_obs_t), INTENT(IN) :: obs
        TYPE(c_ptr), VALUE :: fcst3d, fcst2d, anal3d, anal2d
      END FUNCTION letkf_da_main
+
+     INTEGER(c_int) FUNCTION letkf_da_main_lpfgm(h, grid, obs, cfg, fcst3d, fcst2d, anal3d, anal2d) &
+          BIND(C, name="letkf_da_main_lpfgm")
+       IMPORT :: c_int, c_ptr, letkf_grid_t, letkf_obs_t, letkf_lpfgm_cfg_t
+       TYPE(c_ptr), VALUE :: h
+       TYPE(letkf_grid_t), INTENT(IN) :: grid
+       TYPE(letkf_obs_t), INTENT(IN) :: obs
+       TYPE(letkf_lpfgm_cfg_t), INTENT(IN) :: cfg
+       TYPE(c_ptr), VALUE :: fcst3d, fcst2d, anal3d, anal2d
+     END FUNCTION letkf_da_main_lpfgm
 
      TYPE(c_ptr) FUNCTION letkf_last_error() BIND(C, name="letkf_last_error")
        IMPORT :: c_ptr
@@ -109,12 +127,16 @@ MODULE letkf_da_gpu
 
 CONTAINS
 
-  SUBROUTINE letkf_da_main_gpu(fcst3d, fcst2d, anal3d, anal2d, precision, form)
+  ! rand (nmem, nmonte): the resampling draws when iswitch_da is 2 or 3 (the
+  ! reference draws them with com_rand_seed(nmem, 0, .), common.f90:1066).
+  SUBROUTINE letkf_da_main_gpu(fcst3d, fcst2d, anal3d, anal2d, precision, form, rand)
     USE common_setting
     REAL(r_size), INTENT(IN), TARGET :: fcst3d(nij1,nlev,nmem,nv3d), fcst2d(nij1,nmem,nv2d)
     REAL(r_size), INTENT(OUT), TARGET :: anal3d(nij1,nlev,nmem,nv3d), anal2d(nij1,nmem,nv2d)
     INTEGER, INTENT(IN), OPTIONAL :: precision, form
+    REAL(c_double), INTENT(IN), TARGET, OPTIONAL :: rand(nmem, nmonte)
     TYPE(letkf_params_t) :: p
+    TYPE(letkf_lpfgm_cfg_t) :: lc
     INTEGER(c_int) :: rc
 
     rc = letkf_params_default(p, INT(r_size, c_int32_t))
@@ -148,6 +170,19 @@ CONTAINS
     IF (.NOT. c_associated(handle)) THEN
        rc = letkf_create(p, handle)
        CALL check(rc)
+    END IF
+
+    IF (iswitch_da == 2 .OR. iswitch_da == 3) THEN
+       IF (.NOT. PRESENT(rand)) THEN
+          WRITE(6,'(A)') "***FATAL ERROR: iswitch_da 2/3 needs the resampling draws (rand)"
+          STOP
+       END IF
+       lc%mode = iswitch_da
+       lc%nmonte = nmonte
+       lc%rm_sr = MERGE(1, 0, RM == "SR")
+       lc%dp = MERGE(1, 0, DP)
+       lc%cov_infl_gm = REAL(cov_infl_gm, c_double)
+       lc%rand = c_loc(rand)
     END IF
 
     IF (nobs > 0) THEN
@@ -193,7 +228,12 @@ CONTAINS
       g%phi1 = c_loc(ph)
       g%depth1 = c_loc(dep)
       o%nobsgrd = c_loc(grd)
-      rc = letkf_da_main(handle, g, o, c_loc(fcst3d), c_loc(fcst2d), c_loc(anal3d), c_loc(anal2d))
+      IF (iswitch_da == 2 .OR. iswitch_da == 3) THEN
+         rc = letkf_da_main_lpfgm(handle, g, o, lc, c_loc(fcst3d), c_loc(fcst2d), c_loc(anal3d), &
+              & c_loc(anal2d))
+      ELSE
+         rc = letkf_da_main(handle, g, o, c_loc(fcst3d), c_loc(fcst2d), c_loc(anal3d), c_loc(anal2d))
+      END IF
       CALL check(rc)
     END SUBROUTINE grid_and_run
 

@@ -117,6 +117,7 @@ def test_argument_errors_are_status_codes():
     assert L.letkf_lpfgm_core_batched(None, 3, 1.0, 100, 0, 1, None, 0, *([None] * 7)) == -1
     lc = _lib.LpfgmCfg()
     assert L.letkf_da_main_lpfgm_device(None, C.byref(g), C.byref(o), C.byref(lc), *([None] * 5)) == -1
+    assert L.letkf_da_main_lpfgm(None, C.byref(g), C.byref(o), C.byref(lc), *([None] * 4)) == -1
     # the output-step entries check kinds and sizes before any HIP call
     assert L.letkf_ensemble_mesp_device(3, 10, 2, 4, 1, 1, 999.0, *([None] * 7)) == -1
     assert L.letkf_ensemble_mesp_device(8, 10, 2, 0, 1, 1, 999.0, *([None] * 7)) == -1
@@ -155,9 +156,11 @@ program drv
   use letkf_da_gpu
   implicit none
   real(r_size), allocatable :: f3(:,:,:,:), f2(:,:,:), a3(:,:,:,:), a2(:,:,:)
+  real(8) :: draws(nmem, nmonte)
   nij1 = 1
   allocate(f3(nij1,nlev,nmem,nv3d), f2(nij1,nmem,nv2d), a3(nij1,nlev,nmem,nv3d), a2(nij1,nmem,nv2d))
   if (.false.) call letkf_da_main_gpu(f3, f2, a3, a2)
+  if (.false.) call letkf_da_main_gpu(f3, f2, a3, a2, rand=draws)
 end program drv
 """)
     subprocess.run(["amdflang", "-c", ref, "-o", str(tmp_path / "cs.o")], check=True, cwd=tmp_path)

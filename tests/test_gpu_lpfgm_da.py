@@ -78,3 +78,22 @@ def test_lpf_da_main_rtps_and_no_obs():
     g3, g2, a3o, a2o, _ = _case(8, 3, nobs=0, seed=5)
     np.testing.assert_array_equal(g3, a3o)
     np.testing.assert_array_equal(g2, a2o)
+
+
+def test_lpfgm_host_entry_matches_device():
+    """letkf_da_main_lpfgm (host arrays, what the Fortran driver calls) gives
+    the device entry's analysis bit for bit."""
+    _say("case lpfgm host entry")
+    pr = synth.make_problem(nlon=20, nlat=14, nlev=3, nmem=12, nobs=300, seed=8)
+    pn = pr.numpy()
+    draws = np.random.default_rng(9).uniform(size=(40, 12))
+    eng = engine.Engine(engine.make_params(pn))
+    try:
+        h3, h2 = eng.da_main_host(pn, lpfgm=dict(mode=3, rand=draws, cov_infl_gm=1.1))
+        prd = pr.to("cuda")
+        d3, d2 = eng.da_main_lpfgm_device(prd, draws, mode=3, cov_infl_gm=1.1)
+        torch.cuda.synchronize()
+    finally:
+        eng.close()
+    np.testing.assert_array_equal(h3, synth._to_fortran(d3))
+    np.testing.assert_array_equal(h2, synth._to_fortran(d2))
