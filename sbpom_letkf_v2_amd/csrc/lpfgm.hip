@@ -1,6 +1,8 @@
 // lpfgm.hip -- the local particle filter and its Gaussian-mixture extension
-// (letkf/run/src/common_lpfgm.f90), batched over local problems like
-// letkf_core_batched: letkf_lpfgm_core_batched.
+// (letkf/run/src/common_lpfgm.f90): letkf_lpfgm_core_batched over local
+// problems given in CSR form (k_lpfgm), and the per-point kernel of da_main
+// with iswitch_da 2/3 (k_lpfgm_points), which reads a point's local obs
+// through the search's lists and then updates its ensemble.
 //
 // One workgroup per problem:
 //  * particle weights (pf_weight_gauss_likelihood, :38-77): lane j sums
