@@ -39,19 +39,25 @@ constexpr int kLpfRows = 16;  // W_GM rows staged per pass
 
 struct LpfShared {
   double ll[kLpfMaxK], cw[kLpfMaxK], sq[kLpfMaxK], wb[kLpfMaxK], il[kLpfMaxK];
-  uint32_t cnt[kLpfMaxK * kLpfMaxK / 4];  // byte count per (particle, column)
   double tt[256];
 };
+
+// Dynamic LDS of the LPF kernels: [K*K doubles for the Cholesky work (mode
+// 3)] then the pick counts, one byte per (particle, column), so that small
+// ensembles fit more workgroups per CU.
+__host__ __device__ inline size_t lpf_dyn_bytes(int K, int mode) {
+  return (mode == 3 ? sizeof(double) * (size_t)K * K : 0) + sizeof(uint32_t) * ((size_t)K * K / 4 + 1);
+}
 
 // Particle weights and the pick counts of the nmonte resamplings of one
 // problem with n rows: rows.dx(r, j) = dep - hdxf of row r and member j (in
 // the real kind), rows.rloc(r), rows.rdiag(r).  Ends with a barrier.
 template <typename Rows>
-__device__ void lpf_counts(LpfShared& sh, int K, int n, int nmonte, int dp, const Rows& rows,
+__device__ void lpf_counts(LpfShared& sh, uint32_t* cnt, int K, int n, int nmonte, int dp, const Rows& rows,
                            const double* __restrict__ rtab, const double* __restrict__ ttab) {
 #pragma clang fp contract(off)
   const int t = threadIdx.x;
-  for (int q = t; q < K * K / 4 + 1 && q < kLpfMaxK * kLpfMaxK / 4; q += 256) sh.cnt[q] = 0u;
+  for (int q = t; q < K * K / 4 + 1; q += 256) cnt[q] = 0u;
   for (int q = t; q <= nmonte; q += 256) sh.tt[q] = ttab[q];
   if (t < K) {  // log-likelihood per particle, rows in order, 8 rows of loads in flight
     double s = 0.0;
@@ -107,7 +113,7 @@ __device__ void lpf_counts(LpfShared& sh, int K, int n, int nmonte, int dp, cons
           if (p < 64) u0 |= 1ull << p;
           else u1 |= 1ull << (p - 64);
           const int q = p * K + p;
-          atomicAdd(&sh.cnt[q >> 2], 1u << (8 * (q & 3)));
+          atomicAdd(&cnt[q >> 2], 1u << (8 * (q & 3)));
           prev = p;
         }
       }
@@ -123,7 +129,7 @@ __device__ void lpf_counts(LpfShared& sh, int K, int n, int nmonte, int dp, cons
             if (col < 64) u0 |= 1ull << col;
             else u1 |= 1ull << (col - 64);
             const int q = p * K + col;
-            atomicAdd(&sh.cnt[q >> 2], 1u << (8 * (q & 3)));
+            atomicAdd(&cnt[q >> 2], 1u << (8 * (q & 3)));
           }
         }
         prev = p;
@@ -134,7 +140,7 @@ __device__ void lpf_counts(LpfShared& sh, int K, int n, int nmonte, int dp, cons
         int p = 0;
         while (p < K - 1 && !(x <= sh.cw[p])) ++p;
         const int q = p * K + j;
-        atomicAdd(&sh.cnt[q >> 2], 1u << (8 * (q & 3)));
+        atomicAdd(&cnt[q >> 2], 1u << (8 * (q & 3)));
       }
     }
   }
@@ -142,16 +148,16 @@ __device__ void lpf_counts(LpfShared& sh, int K, int n, int nmonte, int dp, cons
 }
 
 // Entry (i, k) of the averaged resampling matrix W_LPF.
-__device__ __forceinline__ double lpf_w(const LpfShared& sh, int K, int i, int k) {
+__device__ __forceinline__ double lpf_w(const LpfShared& sh, const uint32_t* cnt, int K, int i, int k) {
   const int q = i * K + k;
-  return sh.tt[(sh.cnt[q >> 2] >> (8 * (q & 3))) & 0xffu];
+  return sh.tt[(cnt[q >> 2] >> (8 * (q & 3))) & 0xffu];
 }
 
 // W = W_GM W_LPF into Wb (K x K, row-major), W_GM = alpha Pa + (Pa b) 1^T
 // with Pa = C^-1 = L^-T L^-1 from the Cholesky factor of C (upper triangle
 // in Gb, pitch K) in LDS cm.  Wb may alias Gb.  Ends with a barrier.
 template <typename WT>
-__device__ void gm_times_lpf(LpfShared& sh, double* cm, int K, double alpha, const double* Gb,
+__device__ void gm_times_lpf(LpfShared& sh, const uint32_t* cnt, double* cm, int K, double alpha, const double* Gb,
                              const double* __restrict__ bb, WT* Wb) {
 #pragma clang fp contract(off)
   const int t = threadIdx.x;
@@ -233,12 +239,12 @@ __device__ void gm_times_lpf(LpfShared& sh, double* cm, int K, double alpha, con
       double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
       int j = 0;
       for (; j + 4 <= K; j += 4) {
-        a0 = fma(cm[i * K + j], lpf_w(sh, K, j, k), a0);
-        a1 = fma(cm[i * K + j + 1], lpf_w(sh, K, j + 1, k), a1);
-        a2 = fma(cm[i * K + j + 2], lpf_w(sh, K, j + 2, k), a2);
-        a3 = fma(cm[i * K + j + 3], lpf_w(sh, K, j + 3, k), a3);
+        a0 = fma(cm[i * K + j], lpf_w(sh, cnt, K, j, k), a0);
+        a1 = fma(cm[i * K + j + 1], lpf_w(sh, cnt, K, j + 1, k), a1);
+        a2 = fma(cm[i * K + j + 2], lpf_w(sh, cnt, K, j + 2, k), a2);
+        a3 = fma(cm[i * K + j + 3], lpf_w(sh, cnt, K, j + 3, k), a3);
       }
-      for (; j < K; ++j) a0 = fma(cm[i * K + j], lpf_w(sh, K, j, k), a0);
+      for (; j < K; ++j) a0 = fma(cm[i * K + j], lpf_w(sh, cnt, K, j, k), a0);
       Wb[(long long)(i0 + i) * K + k] = (WT)((a0 + a1) + (a2 + a3));
     }
     __syncthreads();
@@ -264,8 +270,9 @@ __global__ void __launch_bounds__(256) k_lpfgm(int K, int mode, int nmonte, int 
                                                const double* __restrict__ bvec, T* __restrict__ wvec,
                                                T* __restrict__ wmat) {
   __shared__ LpfShared sh;
-  extern __shared__ double cm[];  // K x K (mode 3)
+  extern __shared__ double cm[];  // K x K (mode 3), then the pick counts
   const int b = blockIdx.x, t = threadIdx.x;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cm + (mode == 3 ? K * K : 0));
   const long long o0 = offs[b], o1 = offs[b + 1];
   T* Wb = wmat + (long long)b * K * K;
   if (t < K) wvec[(long long)b * K + t] = (T)0;
@@ -274,12 +281,12 @@ __global__ void __launch_bounds__(256) k_lpfgm(int K, int mode, int nmonte, int 
     return;
   }
   const CsrRows<T> rows{hdxf, rdiag, rloc, dep, o0, K};
-  lpf_counts(sh, K, (int)(o1 - o0), nmonte, dp, rows, rtab, ttab);
+  lpf_counts(sh, cnt, K, (int)(o1 - o0), nmonte, dp, rows, rtab, ttab);
   if (mode == 2) {
-    for (int q = t; q < K * K; q += 256) Wb[q] = (T)lpf_w(sh, K, q / K, q % K);
+    for (int q = t; q < K * K; q += 256) Wb[q] = (T)lpf_w(sh, cnt, K, q / K, q % K);
     return;
   }
-  gm_times_lpf<T>(sh, cm, K, alpha, G + (long long)b * K * K, bvec + (long long)b * K, Wb);
+  gm_times_lpf<T>(sh, cnt, cm, K, alpha, G + (long long)b * K * K, bvec + (long long)b * K, Wb);
 }
 
 // Local obs of one analysis point, as obs_local gathers them
@@ -328,26 +335,27 @@ __global__ void __launch_bounds__(256) k_lpfgm_points(KParams kp, Tables<T> tb, 
   __shared__ LpfShared sh;
   __shared__ T xs[kLpfMaxK], dxf[kLpfMaxK], xa[kLpfMaxK];
   __shared__ T stat[4];  // forecast mean, spread; analysis mean, spread
-  extern __shared__ double cm[];
+  extern __shared__ double cm[];  // K x K (mode 3), then the pick counts
   const int K = kp.K, t = threadIdx.x;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(cm + (la.mode == 3 ? K * K : 0));
   const long long p = p0 + blockIdx.x;
   const int n = nloc[p];
   int wsrc = 0;  // 0 identity, 2 W_LPF, 3 W in the G slot
   double* Wg = nullptr;
   if (n > 0) {
     const LocalRows<T> rows{lobs, lrloc, yT, tb.oerr, tb.odep, loff[p] - lbase, K};
-    lpf_counts(sh, K, n, la.nmonte, la.dp, rows, la.rtab, la.ttab);
+    lpf_counts(sh, cnt, K, n, la.nmonte, la.dp, rows, la.rtab, la.ttab);
     wsrc = 2;
     if (la.mode == 3) {
       const long long a = actidx[p] - a0;
       Wg = G + a * K * K;
-      gm_times_lpf<double>(sh, cm, K, la.alpha, Wg, bvec + a * K, Wg);
+      gm_times_lpf<double>(sh, cnt, cm, K, la.alpha, Wg, bvec + a * K, Wg);
       wsrc = 3;
     }
   }
   auto wij = [&](int i, int j) -> T {
     if (wsrc == 0) return i == j ? (T)1 : (T)0;
-    if (wsrc == 2) return (T)lpf_w(sh, K, i, j);
+    if (wsrc == 2) return (T)lpf_w(sh, cnt, K, i, j);
     return (T)Wg[i * K + j];
   };
   const long long P = kp.P;
@@ -468,7 +476,7 @@ hipError_t launch_lpfgm_points(const KParams& kp, const Tables<T>& tb, const T* 
                                const double* bvec, long long p0, long long p1, const LpfPointArgs& la,
                                hipStream_t s) {
   if (p1 <= p0) return hipSuccess;
-  const size_t lds = la.mode == 3 ? sizeof(double) * (size_t)kp.K * kp.K : 0;
+  const size_t lds = lpf_dyn_bytes(kp.K, la.mode);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_lpfgm_points<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -529,7 +537,7 @@ extern "C" int letkf_lpfgm_core_batched(letkf_handle_t* h, int32_t mode, double 
     bv = h->bvec.as<double>();
     alpha = f64 ? (double)(K - 1) / cov_infl_gm : (double)((float)(K - 1) / (float)cov_infl_gm);
   }
-  const size_t lds = mode == 3 ? sizeof(double) * (size_t)K * K : 0;
+  const size_t lds = lpf_dyn_bytes(K, mode);
   if (lds > 65536) {
     if (f64)
       HIP_TRY(hipFuncSetAttribute((const void*)k_lpfgm<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
