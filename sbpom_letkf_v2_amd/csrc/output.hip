@@ -145,6 +145,8 @@ void launch_mesp(dim3 grid, hipStream_t s, int nij1, int nlev, int nmem, int nv3
     hipLaunchKernelGGL((k_mesp_reg<T, 32>), grid, dim3(256), 0, MESP_ARGS);
   else if (nmem <= 64)
     hipLaunchKernelGGL((k_mesp_reg<T, 64>), grid, dim3(256), 0, MESP_ARGS);
+  else if (nmem <= 104)  // two wavefronts per SIMD in FP64 (the 128 form holds one)
+    hipLaunchKernelGGL((k_mesp_reg<T, 104>), grid, dim3(256), 0, MESP_ARGS);
   else if (nmem <= 128)
     hipLaunchKernelGGL((k_mesp_reg<T, 128>), grid, dim3(256), 0, MESP_ARGS);
   else
