@@ -469,6 +469,14 @@ int letkf_da_adjoint_device(letkf_handle_t* h, const letkf_grid_t* grid,
 int letkf_set_params(letkf_handle_t* h, const letkf_params_t* p);
 
 int letkf_get_stats(const letkf_handle_t* h, letkf_stats_t* out);
+
+/* nobsl of every point -- count_nobsl (common_da.f90:133-205) as the last
+ * analysis, plan or adjoint on the handle counted it -- into the DEVICE
+ * buffer nobsl (nij1,nlev) int32, Fortran order.  The reference keeps the
+ * count per point only inside da_main's loop (common_da.f90:70); this
+ * exposes it for diagnostics and for checking the local-obs selection.
+ * Runs on `stream` (NULL: the handle's) and returns when done. */
+int letkf_get_nobsl(letkf_handle_t* h, int32_t* nobsl, void* stream);
 const char* letkf_last_error(void);
 
 #ifdef __cplusplus

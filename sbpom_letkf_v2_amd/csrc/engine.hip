@@ -120,6 +120,20 @@ extern "C" int letkf_destroy(letkf_handle_t* h) {
 
 extern "C" const char* letkf_last_error(void) { return g_err; }
 
+extern "C" int letkf_get_nobsl(letkf_handle_t* h, int32_t* nobsl, void* stream) {
+  if (!h || !nobsl) return set_err(LETKF_ERR_ARG, "NULL argument");
+  if (h->nloc_P < 0) return set_err(LETKF_ERR_ARG, "letkf_get_nobsl: no analysis or plan on this handle yet");
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  const size_t bytes = sizeof(int32_t) * (size_t)h->nloc_P;
+  if (h->nloc_zero)
+    HIP_TRY(hipMemsetAsync(nobsl, 0, bytes, s));
+  else
+    HIP_TRY(hipMemcpyAsync(nobsl, h->nloc.ptr, bytes, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return LETKF_OK;
+}
+
 extern "C" int letkf_get_stats(const letkf_handle_t* h, letkf_stats_t* out) {
   if (!h || !out) return set_err(LETKF_ERR_ARG, "NULL argument");
   *out = h->stats;
@@ -351,9 +365,13 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   plan_key(g, o, pl.key);
   pl.P = P;
   if (o->nobs == 0) {
+    h->nloc_P = P;
+    h->nloc_zero = true;
     pl.nch = 0;
     pl.st = st;
     pl.valid = true;
+    pl.epoch = ++h->plans_made;
+    h->invalidated_by = nullptr;
     return LETKF_OK;
   }
   HIP_TRY(h->otrig.ensure(sizeof(double2) * (size_t)(o->nobs + 1)));
@@ -372,6 +390,8 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     HIP_TRY(hipEventRecord(h->events[0], s));
   }
   HIP_TRY(launch_search_count<T>(kp, tb, h->nloc.as<int32_t>(), s));
+  h->nloc_P = P;
+  h->nloc_zero = false;
   HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
                              h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
   if (timing) HIP_TRY(hipEventRecord(h->events[1], s));
@@ -502,6 +522,8 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   pl.dual = dual;
   pl.st = st;
   pl.valid = true;
+  pl.epoch = ++h->plans_made;
+  h->invalidated_by = nullptr;
   return LETKF_OK;
 }
 
@@ -593,9 +615,17 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
                       T* a2, hipStream_t s) {
   const letkf_params_t& p = h->prm;
   const bool timing = p.stage_timing != 0;
-  hipEvent_t ev0, ev1;
-  HIP_TRY(hipEventCreate(&ev0));
-  HIP_TRY(hipEventCreate(&ev1));
+  // the two events live until every return below
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } evs;
+  HIP_TRY(hipEventCreate(&evs.e[0]));
+  HIP_TRY(hipEventCreate(&evs.e[1]));
+  hipEvent_t ev0 = evs.e[0], ev1 = evs.e[1];
   HIP_TRY(hipEventRecord(ev0, s));
   int rc = plan_device<T>(h, g, o, s, timing && o->nobs > 0);
   if (rc) return rc;
@@ -628,8 +658,6 @@ static int run_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t*
   }
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
   st.ms_total = ms;
   if (timing && o->nobs > 0) {
     float t = 0.f;
@@ -669,6 +697,8 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     HIP_TRY(hipMemsetAsync(glogvar, 0, sizeof(double) * o->nobs, s));
   }
   if (o->nobs == 0) {  // analysis = forecast (common_da.f90:57-61): no dependence
+    h->nloc_P = P;
+    h->nloc_zero = true;
     HIP_TRY(hipStreamSynchronize(s));
     return LETKF_OK;
   }
@@ -702,6 +732,8 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   HIP_TRY(hipMemsetAsync(h->adj_part.ptr, 0, sizeof(double) * 2 * (size_t)P, s));
   HIP_TRY(launch_transpose_obs<double>(o->nobs, K, (const double*)o->hdxf, h->yT.as<double>(), s));
   HIP_TRY(launch_search_count<double>(kp, tb, h->nloc.as<int32_t>(), s));
+  h->nloc_P = P;
+  h->nloc_zero = false;
   HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
                              h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
   // chunks with a K x K Gram block per active slot (~2 GiB)
@@ -764,13 +796,17 @@ static int run_lpfgm(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* 
   KParams kp = make_kparams(p, o->nobs);
   const long long P = kp.P;
   if (o->nobs == 0) {  // analysis = forecast (common_da.f90:57-61)
+    h->nloc_P = P;
+    h->nloc_zero = true;
     if (p.nv3d) HIP_TRY(hipMemcpyAsync(a3, f3, sizeof(T) * P * K * p.nv3d, hipMemcpyDeviceToDevice, s));
     if (p.nv2d)
       HIP_TRY(hipMemcpyAsync(a2, f2, sizeof(T) * (size_t)p.nij1 * K * p.nv2d, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     return LETKF_OK;
   }
-  LpfPointArgs la{cfg->mode, cfg->nmonte, cfg->dp ? 1 : 0, 0.0, nullptr, nullptr};
+  LpfPointArgs la{cfg->mode, cfg->nmonte, cfg->dp ? 1 : 0, 0.0, nullptr, nullptr, h->status.as<int>()};
+  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, h->status.as<int>());
+  HIP_TRY(hipGetLastError());
   HIP_TRY(lpf_prepare(K, cfg->nmonte, cfg->rm_sr, sizeof(T) == 8, cfg->rand, h->bin_tmp, s, &la.rtab, &la.ttab));
   la.alpha = (double)((T)(K - 1) / (T)cfg->cov_infl_gm);
   KParams kg = kp;
@@ -786,6 +822,8 @@ static int run_lpfgm(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* 
   HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
   HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
   HIP_TRY(launch_search_count<T>(kp, tb, h->nloc.as<int32_t>(), s));
+  h->nloc_P = P;
+  h->nloc_zero = false;
   HIP_TRY(launch_scan_points(P, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), h->actidx.as<int32_t>(),
                              h->act_p.as<int32_t>(), h->scan_tmp.ptr, h->scan_tmp.bytes, s));
   // chunks with a K x K slot per active point (~2 GiB)
@@ -833,8 +871,10 @@ static int run_lpfgm(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* 
                                    h->actidx.as<int32_t>(), a0, cfg->mode == 3 ? h->G.as<double>() : nullptr,
                                    cfg->mode == 3 ? h->bvec.as<double>() : nullptr, p0, p1, la, s));
   }
+  int stat4[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(stat4, h->status.ptr, sizeof(stat4), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  return LETKF_OK;
+  return status_to_rc(stat4[0]);
 }
 
 static int check_call(letkf_handle_t* h, const letkf_grid_t* g, const letkf_obs_t* o, const void* f3,
@@ -878,6 +918,7 @@ extern "C" int letkf_da_main_lpfgm_device(letkf_handle_t* h, const letkf_grid_t*
   if (cfg->mode == 3 && !(cfg->cov_infl_gm > 0.0)) return set_err(LETKF_ERR_ARG, "cov_infl_gm must be > 0");
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  h->invalidate_plan("invalidated by letkf_da_main_lpfgm(_device), which reuses the plan's workspaces");
   if (h->prm.real_kind == 8)
     return run_lpfgm<double>(h, g, o, cfg, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s);
   return run_lpfgm<float>(h, g, o, cfg, (const float*)f3, (const float*)f2, (float*)a3, (float*)a2, s);
@@ -1007,6 +1048,7 @@ extern "C" int letkf_da_main_enqueue(letkf_handle_t* h, const letkf_grid_t* g, c
     return set_err(LETKF_ERR_ARG, "letkf_da_main_enqueue: grid, obs positions or localisation differ from the plan");
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  h->enq_epoch = pl.epoch;
   if (h->prm.real_kind == 8)
     return enqueue_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s, false,
                                   nullptr);
@@ -1021,6 +1063,14 @@ extern "C" int letkf_da_main_status(letkf_handle_t* h, void* stream) {
   int stat4[4] = {0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(stat4, h->status.ptr, sizeof(stat4), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  const AnalysisPlan& pl = h->plan;
+  if (h->enq_epoch != 0 && (!pl.valid || pl.epoch != h->enq_epoch))
+    return set_err(LETKF_ERR_ARG,
+                   "letkf_da_main_status: the plan of the last letkf_da_main_enqueue was %s since; a graph "
+                   "captured from that enqueue refers to workspaces that have been reused and must be captured "
+                   "again after a new letkf_da_main_plan",
+                   pl.valid ? "replaced by a new plan (letkf_da_main_plan, letkf_da_main_device, letkf_autotune)"
+                            : (h->invalidated_by ? h->invalidated_by : "invalidated"));
   return status_to_rc(stat4[0]);
 }
 
@@ -1036,6 +1086,7 @@ extern "C" int letkf_da_adjoint_device(letkf_handle_t* h, const letkf_grid_t* g,
   if (!g_scalars || (o->nobs > 0 && (!g_dep || !g_logvar))) return set_err(LETKF_ERR_ARG, "gradient array is NULL");
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  h->invalidate_plan("invalidated by letkf_da_adjoint_device, which reuses the plan's workspaces");
   return run_adjoint(h, g, o, (const double*)f3, (const double*)f2, (const double*)ga3, (const double*)ga2, g_dep,
                      g_logvar, g_scalars, s);
 }

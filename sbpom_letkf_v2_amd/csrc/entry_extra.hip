@@ -94,6 +94,7 @@ extern "C" int letkf_core_batched(letkf_handle_t* h, int32_t nb, const int64_t* 
   hipStream_t s = h->stream;
   const int K = h->prm.nmem;
   const double rho = h->prm.cov_infl_mul;
+  h->invalidate_plan("invalidated by letkf_core_batched, which may reallocate the plan's workspaces");
   HIP_TRY(h->G.ensure(sizeof(double) * (size_t)nb * K * K));
   HIP_TRY(h->V.ensure(sizeof(double) * (size_t)nb * K * K));
   HIP_TRY(h->lam.ensure(sizeof(double) * (size_t)nb * K));

@@ -61,11 +61,25 @@ struct AnalysisPlan {
   std::vector<long long> bpts, bv;  // chunk point bounds; per-chunk (local obs, active slot) offsets + totals
   std::vector<int32_t> hist, boff;  // per (chunk, size bucket) slot counts and list offsets
   letkf_stats_t st{};               // counters known from the plan
+  uint64_t epoch = 0;               // distinct for every plan made on the handle
 };
 
 struct letkf_handle {
   letkf_params_t prm;
   AnalysisPlan plan;
+  // Graph guard: plans made so far, the plan of the last public enqueue, and
+  // the entry point that last overwrote the plan's workspaces (a graph
+  // captured from an enqueue refers to them).
+  uint64_t plans_made = 0, enq_epoch = 0;
+  const char* invalidated_by = nullptr;
+  // points whose local-obs counts nloc holds (count_nobsl of the last
+  // analysis, plan or adjoint on the handle; -1 none); nloc_zero: no obs
+  long long nloc_P = -1;
+  bool nloc_zero = false;
+  void invalidate_plan(const char* by) {
+    plan.valid = false;
+    invalidated_by = by;
+  }
   int vpath[5] = {0, 0, 0, 0, 0};  // vector path per size bucket (16, 32, 48, 64, 128): letkf_autotune
   int device = 0;
   hipStream_t stream = nullptr;

@@ -117,6 +117,7 @@ struct LpfPointArgs {
   double alpha;  // (K-1)/cov_infl_gm in the real kind
   const double* rtab;
   const double* ttab;
+  int* status;   // status bits: ST_NONFINITE when an analysis value is not finite
 };
 template <typename T>
 hipError_t launch_lpfgm_points(const KParams& kp, const Tables<T>& tb, const T* f3, const T* f2, T* a3, T* a2,

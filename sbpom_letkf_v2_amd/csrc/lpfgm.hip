@@ -429,6 +429,7 @@ __global__ void __launch_bounds__(256) k_lpfgm_points(KParams kp, Tables<T> tb, 
         }
       }
       dst[ms * t] = out;
+      if (!isfinite((double)out)) atomicOr(la.status, ST_NONFINITE);
     }
     __syncthreads();
   }
@@ -515,6 +516,7 @@ extern "C" int letkf_lpfgm_core_batched(letkf_handle_t* h, int32_t mode, double 
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t s = h->stream;
   const bool f64 = h->prm.real_kind == 8;
+  h->invalidate_plan("invalidated by letkf_lpfgm_core_batched, which may reallocate the plan's workspaces");
   const double* d_rt = nullptr;
   const double* d_tt = nullptr;
   HIP_TRY(lpf_prepare(K, nmonte, rm_sr, f64, rand, h->bin_tmp, s, &d_rt, &d_tt));

@@ -48,6 +48,7 @@ struct DualRows {
   const double* yT;
   double alpha;     // (K-1)/rho in r_size
   const int* ovf;   // register path: nonzero once any eigensolve log overflowed
+  int* status;      // status bits (ST_NONFINITE: a non-finite analysis at a point with local obs)
 };
 
 // What one launch of k_transform updates.
@@ -401,6 +402,9 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
       }
       if (h0) xs[((size_t)v * K + m0) * TI + tp] = y0;
       if (h1) xs[((size_t)v * K + m1) * TI + tp] = y1;
+      // NaN / Inf in obsdep, obshdxf or the forecast of a point with local obs
+      if (MODE != TM_NOOBS && !isfinite(wsum((h0 ? y0 : 0.0) + (h1 ? y1 : 0.0))) && lane == 0)
+        atomicOr(dr.status, ST_NONFINITE);
     }
   }
   __syncthreads();
@@ -436,7 +440,9 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0
   const int TI = kp.K <= 64 ? 16 : 8;
   const size_t tile = sizeof(double) * (size_t)nvt * kp.K * TI;
   const size_t lds = tile + sizeof(double) * 4 * (2 * (size_t)(1 + nvt) * kp.K + kp.K / 2 + 1);
-  DualRows dr{dual, fallback_count, loff, lbase, lobs, lrloc, yT, (double)((T)(kp.K - 1) / (T)kp.rho), ovf};
+  // ovf is word 1 of the handle's status block, whose word 0 holds the status bits
+  DualRows dr{dual, fallback_count, loff, lbase, lobs, lrloc, yT, (double)((T)(kp.K - 1) / (T)kp.rho), ovf,
+              const_cast<int*>(ovf) - 1};
   const long long blocks = (p1 - p0 + TI - 1) / TI;
   auto big = [&](auto kern) -> hipError_t {
     if (lds > 65536) {

@@ -93,6 +93,14 @@ class Engine:
             out[n] = list(v) if n in ("ms_eigk", "n_eigk") else v
         return out
 
+    def nobsl(self):
+        """letkf_get_nobsl: per-point local-obs counts of the last analysis
+        or plan, an int32 device tensor (nlev, nij1) (memory = nobsl(nij1,nlev))."""
+        import torch
+        t = torch.empty((int(self.params.nlev), int(self.params.nij1)), dtype=torch.int32, device="cuda")
+        check(_lib.lib().letkf_get_nobsl(self.h, _ptr(t), None))
+        return t
+
     # -------------------------------------------------------- host arrays
     def da_main_host(self, pr, lpfgm=None):
         """letkf_da_main on host arrays (Fortran layouts), or with lpfgm =
@@ -135,14 +143,26 @@ class Engine:
     def _device_structs(self, pr):
         import torch
         dt = torch.float32 if self.params.real_kind == 4 else torch.float64
-        for n in ("lon1", "fcst3d", "obshdxf", "obsdep", "obserr"):
+        p = self.params
+        nij1, nlev, K = int(p.nij1), int(p.nlev), int(p.nmem)
+        nobs = int(pr.obselm.numel())
+        # every array the engine reads: (name, dtype, element count)
+        need = [(n, dt, nij1) for n in ("lon1", "lat1", "ri1", "rj1", "dlon1", "dlat1", "phi1")]
+        need += [("depth1", dt, nij1 * nlev), ("nobsgrd", torch.int32, int(p.nlon) * int(p.nlat)),
+                 ("obselm", torch.int32, nobs)]
+        need += [(n, dt, nobs) for n in ("obslon", "obslat", "obslev", "obserr", "obsdep")]
+        need += [("obshdxf", dt, nobs * K), ("fcst3d", dt, nij1 * nlev * K * int(p.nv3d)),
+                 ("fcst2d", dt, nij1 * K * int(p.nv2d))]
+        for n, t, cnt in need:
             a = getattr(pr, n)
             if not (_is_torch(a) and a.is_cuda):
                 raise TypeError(f"{n} must be a torch tensor on the GPU")
-            if a.dtype != dt:
-                raise TypeError(f"{n} has dtype {a.dtype}, real_kind needs {dt}")
+            if a.dtype != t:
+                raise TypeError(f"{n} has dtype {a.dtype}, needs {t}")
             if not a.is_contiguous():
                 raise ValueError(f"{n} must be contiguous")
+            if a.numel() != cnt:
+                raise ValueError(f"{n} has {a.numel()} elements, the handle's sizes need {cnt}")
         g = _lib.Grid(*(_ptr(getattr(pr, n)) for n in (
             "lon1", "lat1", "ri1", "rj1", "dlon1", "dlat1", "phi1", "depth1")))
         o = _lib.Obs(int(pr.obselm.numel()), _ptr(pr.nobsgrd), _ptr(pr.obselm),

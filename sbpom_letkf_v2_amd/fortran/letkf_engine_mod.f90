@@ -16,7 +16,7 @@ MODULE letkf_engine_mod
   IMPLICIT NONE
   PRIVATE
   PUBLIC :: letkf_params_t, letkf_grid_t, letkf_obs_t, letkf_lpfgm_cfg_t
-  PUBLIC :: letkf_params_default, letkf_create, letkf_destroy, letkf_da_main, letkf_da_main_lpfgm
+  PUBLIC :: letkf_params_default, letkf_create, letkf_destroy, letkf_set_params, letkf_da_main, letkf_da_main_lpfgm
   PUBLIC :: letkf_last_error_f
   PUBLIC :: LETKF_PREC_FP64, LETKF_PREC_FP32, LETKF_PREC_BF16X3, LETKF_PREC_FP32_PANEL, LETKF_PREC_FP32X3, &
             LETKF_PREC_FP32_DUAL64
@@ -69,6 +69,12 @@ MODULE letkf_engine_mod
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: h
      END FUNCTION letkf_destroy
+
+     INTEGER(c_int) FUNCTION letkf_set_params(h, p) BIND(C, name="letkf_set_params")
+       IMPORT :: c_int, c_ptr, letkf_params_t
+       TYPE(c_ptr), VALUE :: h
+       TYPE(letkf_params_t), INTENT(IN) :: p
+     END FUNCTION letkf_set_params
 
      INTEGER(c_int) FUNCTION letkf_da_main(h, grid, obs, fcst3d, fcst2d, anal3d, anal2d) &
           BIND(C, name="letkf_da_main")
@@ -169,6 +175,11 @@ CONTAINS
 
     IF (.NOT. c_associated(handle)) THEN
        rc = letkf_create(p, handle)
+       CALL check(rc)
+    ELSE
+       ! the handle outlives a call: this call's precision, form and
+       ! scalars replace the previous ones (sizes are fixed per run)
+       rc = letkf_set_params(handle, p)
        CALL check(rc)
     END IF
 
