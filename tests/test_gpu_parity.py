@@ -243,17 +243,22 @@ def test_rotation_log_overflow_fallback(monkeypatch):
                                                    ("fp32x3", "batched", 48, 900), ("fp32x3", "batched", 100, 1800),
                                                    ("fp32d", "batched", 48, 900), ("fp32d", "batched", 100, 1800)])
 def test_mixed_precision_gram_close_to_fp64(precision, form, K, nobs):
-    """Gram on MFMA (FP32, FP32 summed per panel, FP32 or BF16 hi/lo splits) with FP64 eigensolve: the
-    analysis must stay within 1e-4 of the FP64 oracle relative to the field
-    scale (FP32 products keep ~7 digits; the innovation term stays FP64).
-    K > 64 runs the 128-wide output tiling, partly filled for K = 100."""
+    """Gram on MFMA (FP32, FP32 summed per panel, FP32 or BF16 hi/lo splits) with FP64 eigensolve
+    against the FP64 oracle: the analysis-mean max error <= 1e-5 relative to
+    the variable's max |mean| (the target bound of the mixed-precision
+    form), every member within 1e-4 of the field scale (FP32 products keep
+    ~7 digits; the innovation term stays FP64).  K > 64 runs the 128-wide
+    output tiling, partly filled for K = 100."""
     _say(f"case {precision}/{form} K={K} nobs={nobs}")
     pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=K, nobs=nobs, seed=61, alpha_rtpp=0.7,
                             real_kind=8, sigma_h_cells=1.0, sigma_v=1.0)
     a3, a2, a3o, a2o, nl, st = _run_both(pr, precision=precision, form=form)
     e3 = per_var_err(a3, a3o, axis_var=3)
-    _say(f"  err3={e3}")
+    m3 = per_var_err(a3.mean(2), a3o.mean(2), axis_var=2)
+    m2 = per_var_err(a2.mean(1), a2o.mean(1), axis_var=1)
+    _say(f"  members err3={max(e3):.2e}; mean err3={m3} err2={m2}")
     assert max(e3) <= 1e-4
+    assert max(m3) <= 1e-5 and max(m2) <= 1e-5
 
 
 def test_c4_cycles_mixed_k():
@@ -261,7 +266,7 @@ def test_c4_cycles_mixed_k():
     over 32, 64, 100, 128, 0.5 .. 4 obs per column, run in sequence on fresh
     handles.  Each cycle's mixed-precision analysis (FP32 and BF16 hi/lo
     Gram alternating) stays within 1e-4 of the FP64 engine on the same
-    cycle; the FP64 engine meets the oracle (1e-10) on the nmem <= 64 cycles."""
+    cycle, its ensemble mean within 1e-5; the FP64 engine meets the oracle (1e-10) on the nmem <= 64 cycles."""
     cycles = synth.c4_cycles(scale=1.0 / 16.0, nlev=6)
     for c, kw in enumerate(cycles):
         prec = "fp32" if c % 2 == 0 else "bf16x3"
@@ -275,8 +280,9 @@ def test_c4_cycles_mixed_k():
             eng.close()
             outs[name] = (synth._to_fortran(a3), synth._to_fortran(a2))
         e3 = per_var_err(outs["mixed"][0], outs["fp64"][0], axis_var=3)
-        _say(f"  mixed vs FP64 err3={max(e3):.2e}")
-        assert max(e3) <= 1e-4
+        m3 = per_var_err(outs["mixed"][0].mean(2), outs["fp64"][0].mean(2), axis_var=2)
+        _say(f"  mixed vs FP64 members err3={max(e3):.2e} mean err3={max(m3):.2e}")
+        assert max(e3) <= 1e-4 and max(m3) <= 1e-5
         if kw["nmem"] <= 64:
             opr = to_oracle(pr.to("cpu"))
             rc, a3o, a2o, nl = po.da_main(opr, dtype=np.float64)
