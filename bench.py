@@ -50,6 +50,11 @@ def parse():
                     help="keep the fixed vector-path rule instead of letkf_autotune on the chosen form")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the plan-once + graph-replay timing of the chosen form")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the BF16 hi/lo Gram variant, the c1 record (K=100 with the tracer, chosen "
+                         "vs plain FP64) and the port-vs-reference CPU calibration of the default run")
+    ap.add_argument("--c1-steps", type=int, default=2,
+                    help="timed steps of each form in the c1 record of the default run")
     ap.add_argument("--adjoint", action="store_true",
                     help="differentiable workload (default c3): time forward + hand-written adjoint and "
                          "check it by central finite differences")
@@ -728,7 +733,7 @@ def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2, lpf
     import pyoracle as po
     nij1, nlev = pr_host_tables.depth1.shape
     P = nij1 * nlev
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores, host = host_cores()
     rng = np.random.default_rng(7)
 
     def sample(npt):
@@ -760,9 +765,74 @@ def cpu_baseline(pr_host_tables, prd, a3_gpu, a2_gpu, target_s, K, nv3, nv2, lpf
     err3 = float(np.abs(g3 - xa3).max() / np.abs(xa3).max())
     surf = pk == nlev - 1
     err2 = float(np.abs(g2[surf] - xa2[surf]).max() / np.abs(xa2[surf]).max()) if (nv2 and surf.any()) else 0.0
-    return dict(value=npt / dt, unit="grid points/s", cores=cores, kind="port",
+    return dict(value=npt / dt, unit="grid points/s", cores=cores, kind="port", host=host,
                 sample=f"{npt} uniformly sampled (i,k) points of the same workload, "
                        f"FP64 C oracle with {cores} OpenMP threads, {dt:.1f} s"), max(err3, err2), npt
+
+
+def host_cores():
+    """Threads for the CPU baseline: every core this process may run on,
+    capped by the host's per-GPU share when the environment sets one
+    (OMP_NUM_THREADS), with the counts it was chosen from."""
+    aff = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = min(aff, share) if share > 0 else aff
+    return cores, {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": share or None,
+                   "threads_used": cores}
+
+
+def core_calibration(seconds=6.0):
+    """The oracle's letkf_core against the reference's own letkf_core
+    (common_letkf.f90:61-210, compiled unmodified: the shipped REAL(4) build
+    at its fixed nmem = 128) on identical local problems, both in the
+    shipped REAL(4) arithmetic, on the same threads: how fast the port that
+    the CPU baseline times runs next to the reference.  None where the
+    reference build is absent."""
+    import concurrent.futures as cf
+    import ctypes as C
+    import numpy as np
+    import pyoracle as po
+    ref = po.ref()
+    if ref is None:
+        return None
+    K = ref.ref_nmem()
+    cores, _ = host_cores()
+    rng = np.random.default_rng(11)
+    sizes = [int(n) for n in rng.choice([40, 120, 300, 600], 64)]
+    probs = []
+    for n in sizes:
+        Y = rng.standard_normal((n, K)).astype(np.float32)
+        Y -= Y.mean(1, keepdims=True)
+        probs.append((np.asfortranarray(Y), (rng.uniform(0.3, 2.0, n) ** 2).astype(np.float32),
+                      rng.uniform(0.01, 1.0, n).astype(np.float32), rng.standard_normal(n).astype(np.float32)))
+    F = C.POINTER(C.c_float)
+
+    def run_ref(pb):
+        Y, rd, rl, dp = pb
+        w = np.zeros(K, np.float32)
+        W = np.zeros((K, K), np.float32, order="F")
+        ref.ref_letkf_core(Y.shape[0], Y.ctypes.data_as(F), rd.ctypes.data_as(F), rl.ctypes.data_as(F),
+                           dp.ctypes.data_as(F), w.ctypes.data_as(F), W.ctypes.data_as(F))
+        return W
+
+    def run_port(pb):
+        Y, rd, rl, dp = pb
+        return po.letkf_core(Y, rd, rl, dp, 1.0, dtype=np.float32)[2]
+
+    out = {}
+    with cf.ThreadPoolExecutor(cores) as ex:
+        for name, fn in (("reference", run_ref), ("port", run_port)):
+            list(ex.map(fn, probs[:cores]))          # warm-up
+            reps, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < seconds / 2 or reps == 0:
+                res = list(ex.map(fn, probs))
+                reps += 1
+            out[name] = (time.perf_counter() - t0) / (reps * len(probs))
+            out[name + "_W"] = res
+    err = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(out["port_W"], out["reference_W"]))
+    return {"nmem": K, "problems": len(probs), "local_obs": sorted(set(sizes)), "threads": cores,
+            "reference_ms_per_problem": 1e3 * out["reference"], "port_ms_per_problem": 1e3 * out["port"],
+            "port_speed_vs_reference": out["reference"] / out["port"], "port_vs_reference_max_rel_diff_W": err}
 
 
 def field_errors(a3, a2, r3, r2, nv3, nv2, K):
@@ -840,9 +910,11 @@ def main():
     log(f"  {sec * 1e3:.1f} ms/step, stages {stage}")
     result_extra = {}
     ref3 = ref2 = None                      # FP64 analysis the errors are measured against
+    plain_sec = None
     if (args.precision, args.form) != ("fp64", "plain") and not args.no_plain:
         log("timing plain FP64 form")
         ref3, ref2, psec, pstage, pst = run_form(prd, "fp64", "plain", max(1, args.steps // 2), 1)
+        plain_sec = psec
         result_extra["plain_fp64"] = {"value": P / psec, "ms_per_step": psec * 1e3,
                                       "stage_ms": {k: v for k, v in pstage.items() if k.startswith("ms_")}}
         result_extra["speedup_vs_plain_fp64"] = psec / sec
@@ -874,16 +946,17 @@ def main():
     kernels = {
         # Gram: 2 n K^2 (primal) or 2 n^2 K (dual) flops per point, one launch per chunk
         gram_kernel: dict(ms=stage["ms_gram"], launches=nch, work=st["gram_flops"], unit="TFLOP/s",
-                          bound="mfma", peak=PEAK_TFLOPS[args.precision]),
-        # Householder tridiagonalisation: (4/3) m^3 flops per point, FP64
+                          bound="valu" if gram_kernel == "k_gram_fp64" else "mfma",
+                          peak=PEAK_TFLOPS[args.precision]),
+        # Householder tridiagonalisation: (4/3) m^3 flops per point, FP64 on the vector ALUs
         "k_eigr_tridiag": dict(ms=tri_ms, launches=tri_n, work=st["eig_flops"], unit="TFLOP/s",
-                               bound="mfma", peak=PEAK_FP64_TFLOPS),
+                               bound="valu", peak=PEAK_FP64_TFLOPS),
         # implicit QL: ~30 FP64 flops per logged rotation (Givens pair, refined rsqrt, shift update)
         "k_eigr_ql": dict(ms=ql_ms, launches=ql_n, work=30.0 * rot, unit="TFLOP/s",
-                          bound="mfma", peak=PEAK_FP64_TFLOPS),
+                          bound="valu", peak=PEAK_FP64_TFLOPS),
         # rotation replay: every logged rotation on the 1+nv3d+nv2d vectors, forward and back, 6 flops each
         "k_eigr_vrot": dict(ms=vr_ms, launches=vr_n, work=12.0 * (1 + nv3 + nv2) * rot, unit="TFLOP/s",
-                            bound="mfma", peak=PEAK_FP64_TFLOPS),
+                            bound="valu", peak=PEAK_FP64_TFLOPS),
         # tile update (points without local obs and overflowed eigensolves): read + write of the ensemble
         "k_transform": dict(ms=stage["ms_transform"], launches=nch, work=st["transform_bytes"],
                             unit="GB/s", bound="hbm", peak=PEAK_HBM_GBS),
@@ -893,6 +966,8 @@ def main():
     dom = kernels[dom_name]
     scale = 1e12 if dom["unit"] == "TFLOP/s" else 1e9
     achieved = dom["work"] / (dom["ms"] * 1e-3) / scale
+    # bound: "hbm" (8 TB/s), "mfma" (the matrix cores at the Gram's operand type) or "valu" (FP64 on the
+    # vector ALUs: 78.6 TFLOP/s on MI355X, the same figure as its FP64 matrix peak)
     roofline = {"bound": dom["bound"], "achieved": achieved, "peak": dom["peak"], "unit": dom["unit"],
                 "frac": achieved / dom["peak"], "traffic": profile_traffic(args, dom_name), "kernel": dom_name,
                 "launches_per_step": dom["launches"], "avg_launch_ms": dom["ms"] / dom["launches"],
@@ -922,6 +997,22 @@ def main():
             cpu, parity, npt = cpu_baseline(opr, prd, ref3, ref2, args.cpu_seconds, K, nv3, nv2)
             log(f"  cpu {cpu['value']:.0f} pts/s on {cpu['cores']} cores; GPU vs oracle max rel err "
                 f"{parity:.2e} over {npt} points")
+    if not args.no_variants and args.workload == "c2":
+        if ref3 is not None and (args.precision, args.form) != ("bf16x3", "batched"):
+            # the configuration names BF16 Gram inputs with a split-precision correction
+            log("timing the BF16 hi/lo-split Gram (batched form)")
+            b3, b2, bsec, bstage, _ = run_form(prd, "bf16x3", "batched", 2, 1)
+            result_extra["bf16x3_batched"] = {
+                "value": P / bsec, "ms_per_step": bsec * 1e3,
+                "speedup_vs_plain_fp64": plain_sec / bsec if plain_sec else None,
+                "errors_vs_fp64": field_errors(b3, b2, ref3, ref2, nv3, nv2, K)["summary"],
+                "stage_ms": {k: v for k, v in bstage.items() if k.startswith("ms_")}}
+            log(f"  {result_extra['bf16x3_batched']}")
+            del b3, b2
+        if not args.no_cpu:
+            log("CPU: the oracle's letkf_core next to the reference's own (REAL(4), nmem 128)")
+            result_extra["cpu_port_vs_reference_core"] = core_calibration()
+            log(f"  {result_extra['cpu_port_vs_reference_core']}")
     out = {
         "metric": METRIC, "value": P / sec, "unit": "grid points/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
@@ -938,7 +1029,50 @@ def main():
         "parity_max_rel_err_vs_fp64_oracle_sample": parity,
     }
     out.update(result_extra)
+    if not args.no_variants and args.workload == "c2":
+        del prd, a3, a2, ref3, ref2
+        torch.cuda.empty_cache()
+        out["c1"] = c1_record(args)
     print(json.dumps(out), flush=True)
+
+
+def c1_record(args):
+    """The target configuration (512x512x40, K=100, 5 3-D variables with
+    the tracer + SSH, 2e5 obs): the chosen form next to plain FP64 on the
+    same problem, the analysis errors between them, and the workload of
+    both forms (sum of m^3 over the eigensolves, Gram flops)."""
+    import torch
+    from sbpom_letkf_v2_amd import synth
+    log("c1 record: generating 512x512x40 K=100 with the tracer on the GPU")
+    prd = synth.config("c1", device="cuda", real_kind=8)
+    torch.cuda.synchronize()
+    m = prd.meta
+    K, nv3, nv2 = m["nmem"], m["nv3d"], m["nv2d"]
+    P = m["nlon"] * m["nlat"] * m["nlev"]
+    a3, a2, sec, stage, st = run_form(prd, args.precision, args.form, args.c1_steps, 1)
+    log(f"  c1 chosen: {sec * 1e3:.1f} ms/step")
+    r3, r2, psec, pstage, pst = run_form(prd, "fp64", "plain", args.c1_steps, 1)
+    log(f"  c1 plain FP64: {psec * 1e3:.1f} ms/step")
+    err = field_errors(a3, a2, r3, r2, nv3, nv2, K)
+    rec = {
+        "workload": f"c1: {m['nlon']}x{m['nlat']}x{m['nlev']} K={K} nv3d={nv3} (tracer) nv2d={nv2} "
+                    f"nobs={m['nobs']} ({m['obs_layout']}), {args.precision}/{args.form}",
+        "value": P / sec, "unit": "grid points/s", "steps": args.c1_steps, "ms_per_step": sec * 1e3,
+        "plain_fp64_value": P / psec, "plain_fp64_ms_per_step": psec * 1e3,
+        "speedup_vs_plain_fp64": psec / sec, "target_speedup": 4.0,
+        "errors_vs_fp64": err,
+        "stage_ms": {k: v for k, v in stage.items() if k.startswith("ms_")},
+        "plain_stage_ms": {k: v for k, v in pstage.items() if k.startswith("ms_")},
+        "eig_kernel_ms": {k: v["ms"] for k, v in stage["eig_kernels"].items()},
+        "work": {"active_points": st["active_points"], "dual_points": st["dual_points"],
+                     "eig_flops_chosen": st["eig_flops"], "eig_flops_plain": pst["eig_flops"],
+                     "gram_flops_chosen": st["gram_flops"], "gram_flops_plain": pst["gram_flops"],
+                     "eig_flops_ratio_plain_vs_chosen": pst["eig_flops"] / max(st["eig_flops"], 1.0)},
+    }
+    log(f"  c1 speedup {psec / sec:.2f}x; errors {err['summary']}")
+    del a3, a2, r3, r2, prd
+    torch.cuda.empty_cache()
+    return rec
 
 
 if __name__ == "__main__":
