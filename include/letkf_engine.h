@@ -233,6 +233,27 @@ int letkf_core_batched(letkf_handle_t* h, int32_t nb, const int64_t* offs,
                        const void* hdxf, const void* rdiag, const void* rloc,
                        const void* dep, void* wvec, void* wmat);
 
+/* letkf_core_batched that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and results as letkf_core_batched, with
+ * its scratch (Gram, eigenvectors, eigenvalues, innovation terms, row
+ * counts, status) in a 64 MiB device array of the library, one per device,
+ * held until the process ends.  Takes no handle: p supplies nmem (2..128),
+ * real_kind (the element type of hdxf, rdiag, rloc, dep, wvec, wmat) and
+ * cov_infl_mul; the other fields are not read.  At most 64 MiB / (16 nmem^2
+ * + 16 nmem + 4) problems per call (1008 at nmem 64, 254 at nmem 128);
+ * more return LETKF_ERR_CAPACITY.  Work is only enqueued on `stream` (NULL:
+ * the null stream); the call returns at once.  *status (DEVICE int32)
+ * receives the letkf_status_t of the problems (0, LETKF_ERR_EIG or
+ * LETKF_ERR_NONFINITE) when the work completes.  Calls on one device share
+ * the scratch: calls that overlap in time on one device race. */
+int letkf_core_batched_noalloc(const letkf_params_t* p, int32_t nb,
+                               const int64_t* offs, const void* hdxf,
+                               const void* rdiag, const void* rloc,
+                               const void* dep, void* wvec, void* wmat,
+                               int32_t* status, void* stream);
+
 /* obs_id_uniform (common.f90:112-168) and the cell sort of set_obs
  * (common_obs.f90:266-377) on DEVICE arrays: lon/lat the global (nlon,nlat)
  * grid, olon/olat the nobs unsorted positions.  Writes idx/idy (1-based

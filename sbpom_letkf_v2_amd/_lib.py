@@ -27,7 +27,8 @@ EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_setobs_default", "letkf_set_obs_device", "letkf_da_main_plan", "letkf_da_main_enqueue",
            "letkf_da_main_status", "letkf_autotune", "letkf_set_tuning", "letkf_get_tuning",
            "letkf_ensemble_mesp_device", "letkf_grid_pack_device", "letkf_lpfgm_core_batched",
-           "letkf_da_main_lpfgm_device", "letkf_da_main_lpfgm", "letkf_get_nobsl")
+           "letkf_da_main_lpfgm_device", "letkf_da_main_lpfgm", "letkf_get_nobsl",
+           "letkf_core_batched_noalloc")
 
 
 class LpfgmCfg(C.Structure):
@@ -139,6 +140,7 @@ def lib():
                                             C.POINTER(C.c_int32)] + [vp] * 11 + [C.POINTER(ObsSet)])
         L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.letkf_get_nobsl.argtypes = [vp, vp, vp]
+        L.letkf_core_batched_noalloc.argtypes = [C.POINTER(Params), C.c_int32] + [vp] * 9
         i32 = C.c_int32
         L.letkf_ensemble_mesp_device.argtypes = [i32] * 6 + [C.c_double] + [vp] * 7
         L.letkf_da_main_lpfgm_device.argtypes = [vp, C.POINTER(Grid), C.POINTER(Obs), C.POINTER(LpfgmCfg),

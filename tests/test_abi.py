@@ -109,6 +109,7 @@ def test_argument_errors_are_status_codes():
     assert L.letkf_bin_obs(None, 8, 6, None, None, 0, None, None, None, None, None, None, None) == -1
     assert L.letkf_aoei(None, 0, None, None, None) == -1
     assert L.letkf_get_nobsl(None, None, None) == -1
+    assert L.letkf_core_batched_noalloc(None, 0, *([None] * 9)) == -1
     assert L.letkf_core_batched(None, 0, None, None, None, None, None, None, None) == -1
     assert L.letkf_da_main_device(None, C.byref(g), C.byref(o), None, None, None, None, None) == -1
     assert L.letkf_da_adjoint_device(None, C.byref(g), C.byref(o), *([None] * 8)) == -1
