@@ -82,6 +82,10 @@ typedef struct letkf_params {
   int32_t stage_timing;         /* 1: time each stage with HIP events into */
                                 /* letkf_stats_t (adds event records only); */
                                 /* 2: also each eigensolve kernel launch    */
+  int64_t workspace_bytes;      /* cap on the analysis workspaces a plan   */
+                                /* sizes its point chunks by; 0: 45% of    */
+                                /* the device memory free at the plan (plus */
+                                /* what the handle holds), at most 96 GB    */
 } letkf_params_t;
 
 /* Local grid of the analysis (common_setting.f90:168-176). */

@@ -44,7 +44,7 @@ class Params(C.Structure):
         (n, C.c_double) for n in (
             "pi", "re", "cov_infl_mul", "alpha_rtpp", "alpha_rtps",
             "sigma_obs", "sigma_obsv", "dist_zero", "dist_zerov", "tmin",
-            "tmax", "smin", "smax")] + [("stage_timing", C.c_int32)]
+            "tmax", "smin", "smax")] + [("stage_timing", C.c_int32), ("workspace_bytes", C.c_int64)]
 
 
 class Grid(C.Structure):

@@ -45,6 +45,7 @@ static int check_params(const letkf_params_t* p) {
   if (p->nv3d < 0 || p->nv2d < 0 || p->nv3d + p->nv2d > 8 || p->nv3d + p->nv2d < 1)
     return set_err(LETKF_ERR_ARG, "nv3d+nv2d must be in [1,8] (got %d+%d)", p->nv3d, p->nv2d);
   if (!(p->cov_infl_mul > 0.0)) return set_err(LETKF_ERR_ARG, "cov_infl_mul must be > 0");
+  if (p->workspace_bytes < 0) return set_err(LETKF_ERR_ARG, "workspace_bytes must be >= 0");
   if (p->precision != LETKF_PREC_FP64 && p->precision != LETKF_PREC_FP32 && p->precision != LETKF_PREC_BF16X3 &&
       p->precision != LETKF_PREC_FP32_PANEL && p->precision != LETKF_PREC_FP32X3 &&
       p->precision != LETKF_PREC_FP32_DUAL64)
@@ -413,7 +414,9 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     const double held = (double)(h->G.bytes + h->V.bytes + h->yvec.bytes + h->lam.bytes + h->bvec.bytes +
                                  h->eD.bytes + h->eE.bytes + h->logc.bytes + h->logs.bytes + h->logi.bytes +
                                  h->logn.bytes + h->logx.bytes + h->logr.bytes);
-    const double budget = std::min(96e9, std::max(1e9, 0.45 * ((double)mfree + held)));
+    double budget = std::min(96e9, std::max(1e9, 0.45 * ((double)mfree + held)));
+    if (p.workspace_bytes > 0)  // the caller's cap (room for several handles, or for the caller's own data)
+      budget = std::min(std::max(1e9, (double)p.workspace_bytes), 0.9 * ((double)mfree + held));
     PC = (long long)(budget / per_slot) / 256 * 256;  // multiple of 256 (k_bucket_scatter)
   } else {
     PC = (long long)((1ULL << 31) / (sizeof(double) * (size_t)K * K));

@@ -33,6 +33,7 @@ MODULE letkf_engine_mod
      REAL(c_double) :: sigma_obs, sigma_obsv, dist_zero, dist_zerov
      REAL(c_double) :: tmin, tmax, smin, smax
      INTEGER(c_int32_t) :: stage_timing
+     INTEGER(c_int64_t) :: workspace_bytes
   END TYPE letkf_params_t
 
   TYPE, BIND(C) :: letkf_grid_t

@@ -39,7 +39,7 @@ def test_struct_layouts_match_c(tmp_path):
 #include "letkf_engine.h"
 int main(void) {
   printf("%zu %zu %zu %zu\n", sizeof(letkf_params_t), sizeof(letkf_grid_t), sizeof(letkf_obs_t), sizeof(letkf_stats_t));
-  printf("%zu %zu %zu\n", offsetof(letkf_params_t, pi), offsetof(letkf_params_t, smax), offsetof(letkf_params_t, stage_timing));
+  printf("%zu %zu %zu %zu\n", offsetof(letkf_params_t, pi), offsetof(letkf_params_t, smax), offsetof(letkf_params_t, stage_timing), offsetof(letkf_params_t, workspace_bytes));
   printf("%zu %zu\n", offsetof(letkf_obs_t, nobsgrd), offsetof(letkf_stats_t, ms_total));
   printf("%zu %zu %zu %zu\n", sizeof(letkf_xtoy_grid_t), offsetof(letkf_xtoy_grid_t, hnosea),
          sizeof(letkf_setobs_cfg_t), offsetof(letkf_setobs_cfg_t, qc_max));
@@ -52,6 +52,7 @@ int main(void) {
     got = [int(x) for x in out]
     want = [C.sizeof(_lib.Params), C.sizeof(_lib.Grid), C.sizeof(_lib.Obs), C.sizeof(_lib.Stats),
             _lib.Params.pi.offset, _lib.Params.smax.offset, _lib.Params.stage_timing.offset,
+            _lib.Params.workspace_bytes.offset,
             _lib.Obs.nobsgrd.offset, _lib.Stats.ms_total.offset,
             C.sizeof(_lib.XtoyGrid), _lib.XtoyGrid.hnosea.offset, C.sizeof(_lib.SetObsCfg),
             _lib.SetObsCfg.qc_max.offset, C.sizeof(_lib.ObsSet), _lib.ObsSet.nobs_out.offset]

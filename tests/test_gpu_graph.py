@@ -122,3 +122,54 @@ def test_graph_replays_match_eager(real_kind):
     finally:
         eng.close()
         eager.close()
+
+
+def test_c4_eight_cycles_in_one_graph():
+    """Workload c4 at 1/16 of its horizontal size: the eight independent
+    cycles (nmem 32/64/100/128, 0.5 .. 4 obs per column, REAL(4) I/O) each
+    planned on its own handle (workspace cap 1 GB), their enqueues captured
+    in ONE graph on one stream; three replays, with new forecast and
+    innovation values and eager analyses between them, reproduce each
+    cycle's eager analysis bit for bit."""
+    cycles = synth.c4_cycles(scale=1.0 / 16.0, nlev=6)
+    probs, engs, outs = [], [], []
+    try:
+        for c, kw in enumerate(cycles):
+            pr = synth.make_problem(device="cuda", real_kind=4, sigma_h_cells=1.5, sigma_v=1.0, **kw)
+            prm = engine.make_params(pr, "fp32" if c % 2 == 0 else "bf16x3", "batched")
+            prm.workspace_bytes = 1 << 30
+            eng = engine.Engine(prm)
+            eng.plan_device(pr)
+            probs.append(pr)
+            engs.append(eng)
+            outs.append((torch.empty_like(pr.fcst3d), torch.empty_like(pr.fcst2d)))
+        _say(f"case c4 in one graph: {len(cycles)} cycles planned")
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, capture_error_mode="thread_local"):
+            s = torch.cuda.current_stream().cuda_stream
+            for eng, pr, (a3, a2) in zip(engs, probs, outs):
+                eng.enqueue_device(pr, a3, a2, stream=s)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(21)
+        for rep in range(3):
+            for pr in probs:
+                pr.fcst3d.add_((0.02 * torch.randn(pr.fcst3d.shape, generator=gen, device="cuda")).to(pr.fcst3d.dtype))
+                pr.obsdep.mul_(1.05)
+            g.replay()
+            torch.cuda.synchronize()
+            for c, (eng, pr, (a3, a2)) in enumerate(zip(engs, probs, outs)):
+                eng.status()
+                fresh = engine.Engine(engine.make_params(pr, "fp32" if c % 2 == 0 else "bf16x3", "batched"))
+                try:
+                    w3, w2 = fresh.da_main_device(pr)
+                    torch.cuda.synchronize()
+                finally:
+                    fresh.close()
+                same = torch.equal(a3, w3) and torch.equal(a2, w2)
+                if not same:
+                    _say(f"  replay {rep} cycle {c}: max |graph - eager| {(a3 - w3).abs().max().item():.3e}")
+                assert same
+            _say(f"  replay {rep}: all {len(cycles)} cycles bit-exact")
+    finally:
+        for eng in engs:
+            eng.close()
