@@ -210,12 +210,150 @@ __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, 
   if (t == 0 && nrow) nrow[blockIdx.x] = n;
 }
 
+// ------------------------------------------------------------------------
+// The same FP64 products on the FP64 matrix cores (v_mfma_f64_16x16x4f64):
+// the output (K x K, or n x n in the dual form) in 16 x 16 tiles, only the
+// T (T + 1) / 2 tiles that hold part of the upper triangle (T = OUT / 16),
+// dealt round-robin to the four wavefronts (3 / 3 / 2 / 2 of 10 at OUT 64,
+// 9 each of 36 at OUT 128); the summation index (obs, or members in the
+// dual form) staged through LDS in panels of 16 rows, four rows per MFMA.
+// Lane layout (checked on gfx950): a = A[i][k], b = B[k][j] with
+// i, j = lane % 16, k = lane / 16; D[i][j] in lane j + 16 (i % 4), element
+// i / 4.  Operands and accumulation are FP64, as in k_gram_fp64.
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+constexpr int kP64 = 16;  // summation rows per LDS panel
+
+template <int OUT, typename Src>
+__global__ void __launch_bounds__(256) k_gram_mfma64(int K, double diag, Src src, double* __restrict__ G,
+                                                     double* __restrict__ bvec, int32_t* __restrict__ nrow,
+                                                     int dual) {
+  constexpr int T = OUT / 16;
+  constexpr int NU = T * (T + 1) / 2;
+  constexpr int NT = (NU + 3) / 4;
+  constexpr int LP = OUT + 1;            // odd pitch: row and column walks hit distinct banks
+  constexpr int kStage = kP64 * OUT / 256;  // panel elements staged per thread (4 or 8)
+  __shared__ double As[kP64][LP];        // primal: Rlinv_dYf rows; dual: B^T panel (A = B)
+  __shared__ double Bs[kP64][LP];        // primal: hdxf rows
+  __shared__ double ds[kP64];
+  __shared__ double sws[OUT];
+  __shared__ int ios[OUT];
+  const int n = src.begin(blockIdx.x);
+  const bool dm = dual && n < K;
+  if (dual == 2 && !dm) return;  // dual slots only (the FP32 MFMA kernel takes the others)
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int N = dm ? n : K;  // output dimension
+  const int S = dm ? K : n;  // summation length
+  // this wavefront's upper-triangle tiles u = wv, wv + 4, ... (row-major over ti <= tj)
+  int tr[NT], tc[NT];
+  bool live[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    const int u = wv + 4 * q;
+    int ti = 0, rem = u;
+    while (ti < T && rem >= T - ti) {
+      rem -= T - ti;
+      ++ti;
+    }
+    tr[q] = 16 * ti;
+    tc[q] = 16 * (ti + rem);
+    live[q] = u < NU && tr[q] < N && tc[q] < N;
+  }
+  mf_d4 acc[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) acc[q] = mf_d4{0.0, 0.0, 0.0, 0.0};
+  if (dm) {
+    for (int l = t; l < n; l += 256) {
+      int io;
+      double sw;
+      src.brow(l, io, sw);
+      ios[l] = io;
+      sws[l] = sw;
+      bvec[(long long)blockIdx.x * K + l] = sw * src.dep(l);
+    }
+    __syncthreads();
+  }
+  double bacc = 0.0;  // primal: b = Rlinv_dYf^T dep, thread t = member t
+  for (int s0 = 0; s0 < S; s0 += kP64) {
+    const int ps = min(kP64, S - s0);
+    if (dm) {
+      // B^T panel: lanes walk 16 consecutive members of one obs (contiguous in yT)
+      double yv[kStage];
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int q = t + 256 * u, mm = q & (kP64 - 1), l = q / kP64;
+        yv[u] = 0.0;
+        if (mm < ps && l < N) src.row_of(ios[l], K, s0 + mm, yv[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int q = t + 256 * u, mm = q & (kP64 - 1), l = q / kP64;
+        As[mm][l] = l < N ? sws[l] * yv[u] : 0.0;
+      }
+    } else {
+      // Y panel: lanes walk the members of one obs, four loads in flight per batch
+      constexpr int kB = 4;
+#pragma unroll
+      for (int u0 = 0; u0 < kStage; u0 += kB) {
+        double yv[kB], rl[kB], rd[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+          const int q = t + 256 * (u0 + u), l = q / OUT, m = q % OUT;
+          yv[u] = 0.0;
+          rl[u] = 0.0;
+          rd[u] = 1.0;
+          if (l < ps && m < K) src.row(s0 + l, K, m, yv[u], rl[u], rd[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+          const int q = t + 256 * (u0 + u), l = q / OUT, m = q % OUT;
+          As[l][m] = yv[u] * rl[u] / rd[u];  // Rlinv_dYf = hdxf * rloc / rdiag (common_letkf.f90:95)
+          Bs[l][m] = yv[u];
+        }
+      }
+      if (t < kP64) ds[t] = t < ps ? src.dep(s0 + t) : 0.0;
+    }
+    __syncthreads();
+    const double(*Xs)[LP] = As;
+    const double(*Ys)[LP] = dm ? As : Bs;
+    for (int k0 = 0; k0 < ps; k0 += 4) {
+      const int kk = k0 + (lane >> 4);
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        if (!live[q]) continue;
+        const double x = Xs[kk][tr[q] + (lane & 15)];
+        const double y = Ys[kk][tc[q] + (lane & 15)];
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[q], 0, 0, 0);
+      }
+    }
+    if (!dm && t < K)
+      for (int l = 0; l < ps; ++l) bacc = fma(As[l][t], ds[l], bacc);
+    __syncthreads();
+  }
+  double* Gp = G + (long long)blockIdx.x * K * K;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    if (!live[q]) continue;
+    const int col = tc[q] + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = tr[q] + 4 * r + (lane >> 4);
+      if (row < N && col < N && row <= col)  // upper triangle: all its readers use
+        Gp[row * N + col] = acc[q][r] + (!dm && row == col ? diag : 0.0);
+    }
+  }
+  if (!dm && t < K) bvec[(long long)blockIdx.x * K + t] = bacc;
+  if (t == 0 && nrow) nrow[blockIdx.x] = n;
+}
+
 template <typename Src>
 static hipError_t launch_gram_src(int K, double diag, const Src& src, int na, double* G, double* bvec,
                                   int32_t* nrow, int dual, hipStream_t s) {
   if (na <= 0) return hipSuccess;
   if (K <= 64)
-    hipLaunchKernelGGL((k_gram_fp64<64, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow, dual);
+    hipLaunchKernelGGL((k_gram_mfma64<64, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow, dual);
+  else if (K <= 128)
+    hipLaunchKernelGGL((k_gram_mfma64<128, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow, dual);
   else
     hipLaunchKernelGGL((k_gram_fp64<128, Src>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec, nrow, dual);
   return hipGetLastError();
