@@ -471,16 +471,22 @@ int letkf_grid_pack_device(int32_t real_kind, int32_t nlon, int32_t nlat,
 
 /* Adjoint of letkf_da_main_device (no counterpart in the reference; for
  * gradient-based uses of the analysis: observation impact, tuning of the
- * inflation and of the observation errors, torch.autograd).  DEVICE arrays;
- * grid, obs, fcst and the handle's parameters must be those of the forward
- * call.  Given ga3/ga2 = dL/d(anal3d, anal2d), writes
+ * inflation, of the observation errors and of the localisation,
+ * torch.autograd).  DEVICE arrays; grid, obs, fcst and the handle's
+ * parameters must be those of the forward call.  Given ga3/ga2 =
+ * dL/d(anal3d, anal2d), writes
  *   g_dep[nobs]    dL/d obs->dep          (innovations, i.e. obs values)
  *   g_logvar[nobs] dL/d log(obs->err^2)   (per-obs log error variance)
- *   g_scalars[2]   dL/d alpha_rtpp, dL/d cov_infl_mul.
+ *   g_scalars[4]   dL/d alpha_rtpp, dL/d cov_infl_mul, dL/d sigma_obs,
+ *                  dL/d sigma_obsv (the localisation scales of rloc,
+ *                  common_da.f90:282-287, with the cut-offs dist_zero and
+ *                  dist_zerov and so the local obs sets held fixed; 0 for
+ *                  sigma_obsv <= 0).
  * The per-point eigen-decompositions are recomputed (FP64, K x K form);
- * round_off clamps pass no gradient.  Needs real_kind 8, nmem <= 64 and
- * alpha_rtps = 0.  Work is queued on `stream` (NULL: the handle's stream)
- * and the call returns after it completes. */
+ * round_off clamps pass no gradient.  Every sum is formed in a fixed
+ * order (no atomics): the gradients are the same bits on every run.  Needs
+ * real_kind 8, nmem <= 64 and alpha_rtps = 0.  Work is queued on `stream`
+ * (NULL: the handle's stream) and the call returns after it completes. */
 int letkf_da_adjoint_device(letkf_handle_t* h, const letkf_grid_t* grid,
                             const letkf_obs_t* obs, const void* fcst3d,
                             const void* fcst2d, const void* ga3,

@@ -15,6 +15,8 @@
 // evaluated in every lane.
 #include <cstdlib>
 
+#include <hipcub/hipcub.hpp>
+
 #include "letkf_internal.h"
 #include "wave.h"
 

@@ -694,7 +694,7 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   const int K = p.nmem;
   KParams kp = make_kparams(p, o->nobs);
   const long long P = kp.P;
-  HIP_TRY(hipMemsetAsync(gsc, 0, 2 * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(gsc, 0, 4 * sizeof(double), s));
   if (o->nobs > 0) {
     HIP_TRY(hipMemsetAsync(gdep, 0, sizeof(double) * o->nobs, s));
     HIP_TRY(hipMemsetAsync(glogvar, 0, sizeof(double) * o->nobs, s));
@@ -730,9 +730,9 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   HIP_TRY(h->act_p.ensure(sizeof(int32_t) * (P + 1)));
   HIP_TRY(h->scan_tmp.ensure(scan_tmp_bytes(P)));
   HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
-  HIP_TRY(h->adj_part.ensure(sizeof(double) * 2 * (size_t)P));
+  HIP_TRY(h->adj_part.ensure(sizeof(double) * (4 * (size_t)P + 1024)));
   HIP_TRY(hipMemsetAsync(h->status.ptr, 0, sizeof(int) * 4, s));
-  HIP_TRY(hipMemsetAsync(h->adj_part.ptr, 0, sizeof(double) * 2 * (size_t)P, s));
+  HIP_TRY(hipMemsetAsync(h->adj_part.ptr, 0, sizeof(double) * 4 * (size_t)P, s));
   HIP_TRY(launch_transpose_obs<double>(o->nobs, K, (const double*)o->hdxf, h->yT.as<double>(), s));
   HIP_TRY(launch_search_count<double>(kp, tb, h->nloc.as<int32_t>(), s));
   h->nloc_P = P;
@@ -764,9 +764,18 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   HIP_TRY(h->lrloc.ensure(sizeof(double) * (maxL + 1)));
   HIP_TRY(h->G.ensure(sizeof(double) * (size_t)(maxA + 1) * K * K));
   HIP_TRY(h->bvec.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
+  // deterministic per-obs sums: pair terms, (obs, position) sort keys and values, sort scratch
+  int nobs_bits = 1;
+  while (nobs_bits < 31 && (1LL << nobs_bits) <= o->nobs) ++nobs_bits;
+  const size_t sort_tmp = adjoint_sort_tmp_bytes(maxL, nobs_bits);
+  HIP_TRY(h->adj_pair.ensure(sizeof(double) * 2 * (size_t)(maxL + 1)));
+  HIP_TRY(h->adj_sort.ensure(sizeof(int32_t) * 3 * (size_t)(maxL + 1) + sort_tmp + 256));
+  int32_t* kv = h->adj_sort.as<int32_t>();
+  void* sort_buf = (void*)(((uintptr_t)(kv + 3 * (maxL + 1)) + 255) & ~(uintptr_t)255);
   for (long long c = 0; c < nch; ++c) {
     const long long p0 = bpts[c], p1 = bpts[c + 1];
     const long long lbase = bv[2 * c];
+    const long long nl = bv[2 * (c + 1)] - lbase;
     const int a0 = (int)bv[2 * c + 1];
     const int na = (int)(bv[2 * (c + 1) + 1] - a0);
     if (na > 0) {
@@ -779,9 +788,12 @@ static int run_adjoint(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     HIP_TRY(launch_adjoint_points(kp, tb, f3, f2, b3, b2, h->nloc.as<int32_t>(), h->loff.as<int64_t>(), lbase,
                                   h->lobs.as<int32_t>(), h->lrloc.as<double>(), h->yT.as<double>(),
                                   h->G.as<double>(), h->bvec.as<double>(), h->act_p.as<int32_t>(), a0, na, p0, p1,
-                                  gdep, glogvar, h->adj_part.as<double>(), h->status.as<int>(), s));
+                                  h->adj_pair.as<double>(), h->adj_part.as<double>(), h->status.as<int>(), s));
+    if (na > 0)
+      HIP_TRY(launch_adjoint_pairs(nl, nobs_bits, h->lobs.as<int32_t>(), h->adj_pair.as<double>(), kv, sort_buf,
+                                   sort_tmp, gdep, glogvar, s));
   }
-  HIP_TRY(launch_adjoint_reduce(h->adj_part.as<double>(), P, gsc, s));
+  HIP_TRY(launch_adjoint_reduce(h->adj_part.as<double>(), P, h->adj_part.as<double>() + 4 * (size_t)P, gsc, s));
   int status = 0;
   HIP_TRY(hipMemcpyAsync(&status, h->status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));

@@ -88,7 +88,8 @@ struct letkf_handle {
   DevBuf eD, eE, logc, logs, logi, logn, logx, logr, yvec;  // eigensolve workspaces
   DevBuf rotsum;  // stage_timing 2: logged QL rotations, summed over slots
   DevBuf otrig;   // (sin, cos) of every obs latitude
-  DevBuf adj_part;  // adjoint: per-point (dL/dalpha, dL/drho)
+  DevBuf adj_part;  // adjoint: per-point (dL/dalpha, dL/drho, dL/dsigma_obs, dL/dsigma_obsv), block sums
+  DevBuf adj_pair, adj_sort;  // adjoint: per-pair terms; sort keys, values and scratch
   DevBuf bhist, bcur, blist;              // per-chunk size buckets of active slots
   // staging for the host-pointer entry point
   DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
@@ -116,7 +117,7 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &adj_part, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();

@@ -192,16 +192,22 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
 
 // Adjoint of the analysis (FP64, K <= 64): the active slots [0,na) of a
 // chunk (C and b from launch_gram_fp64 in the K x K form) and the points
-// [p0,p1) without local obs add their gradients to gdep / glogvar (per obs,
-// atomically) and write (dL/dalpha, dL/drho) per point into part.
+// [p0,p1) without local obs: each (point, local obs) pair writes its
+// (dL/d dep, dL/d log r) terms into gpair (2 per local obs of the chunk),
+// each point (dL/dalpha, dL/drho, dL/dsigma_obs, dL/dsigma_obsv) into part.
 hipError_t launch_adjoint_points(const KParams& kp, const Tables<double>& tb, const double* X3, const double* X2,
                                  const double* B3, const double* B2, const int32_t* nloc, const int64_t* loff,
                                  long long lbase, const int32_t* lobs, const double* lrloc, const double* yT,
                                  const double* G, const double* bvec, const int32_t* act_p, int a0, int na,
-                                 long long p0, long long p1, double* gdep, double* glogvar, double* part,
-                                 int* status, hipStream_t s);
-// out[0..1] += sums over points of part (see launch_adjoint_points).
-hipError_t launch_adjoint_reduce(const double* part, long long P, double* out, hipStream_t s);
+                                 long long p0, long long p1, double* gpair, double* part, int* status,
+                                 hipStream_t s);
+// gdep / glogvar += the chunk's pair terms summed per obs in point order
+// (kv: 3 nl int32 of scratch, tmp: adjoint_sort_tmp_bytes).
+hipError_t launch_adjoint_pairs(long long nl, int nobs_bits, const int32_t* lobs, const double* gpair, int32_t* kv,
+                                void* tmp, size_t tmp_bytes, double* gdep, double* glogvar, hipStream_t s);
+size_t adjoint_sort_tmp_bytes(long long nl, int nobs_bits);
+// out[0..3] = sums over the P points of part (blk: 1024 doubles of scratch).
+hipError_t launch_adjoint_reduce(const double* part, long long P, double* blk, double* out, hipStream_t s);
 
 // Weights + ensemble transform + RTPP/RTPS + round_off for points [p0,p1).
 template <typename T>

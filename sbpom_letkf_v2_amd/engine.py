@@ -314,15 +314,16 @@ class Engine:
 
     def da_adjoint_device(self, pr, g3, g2, stream=None):
         """letkf_da_adjoint_device: (dL/d obsdep, dL/d log obserr^2,
-        [dL/d alpha_rtpp, dL/d cov_infl_mul]) for upstream gradients g3, g2 of
-        (anal3d, anal2d); pr and the handle's parameters as in the forward call."""
+        [dL/d alpha_rtpp, dL/d cov_infl_mul, dL/d sigma_obs, dL/d sigma_obsv])
+        for upstream gradients g3, g2 of (anal3d, anal2d); pr and the
+        handle's parameters as in the forward call."""
         import torch
         g, o = self._device_structs(pr)
         dev = pr.fcst3d.device
         nobs = int(pr.obselm.numel())
         gd = torch.zeros(nobs, dtype=torch.float64, device=dev)
         gl = torch.zeros(nobs, dtype=torch.float64, device=dev)
-        gs = torch.zeros(2, dtype=torch.float64, device=dev)
+        gs = torch.zeros(4, dtype=torch.float64, device=dev)
         g3 = g3.contiguous()
         g2 = g2.contiguous() if g2 is not None else None
         s = C.c_void_p(stream) if stream else None

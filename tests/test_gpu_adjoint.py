@@ -2,7 +2,10 @@
 central finite differences in FP64 and an independent FP64 reference
 gradient (a dense torch restatement of letkf_core + RTPP over the oracle's
 local observation sets, differentiated by torch.autograd through
-torch.linalg.eigh on the CPU)."""
+torch.linalg.eigh on the CPU), with respect to the innovations, the per-obs
+log error variances, RTPP alpha, rho and the localisation scales
+sigma_obs / sigma_obsv (cut-offs held fixed); and run-to-run bit equality
+of the gradients (no atomics in the adjoint)."""
 import numpy as np
 import pytest
 import torch
@@ -27,9 +30,12 @@ def _problem(K=16, nobs=400, seed=11, nv2d=1, alpha=0.7):
 def _params(pr):
     dep = pr.obsdep.clone()
     logvar = torch.log(pr.obserr.clone() ** 2)
-    alpha = torch.tensor(float(pr.params["alpha_rtpp"]), dtype=torch.float64, device="cuda")
-    rho = torch.tensor(float(pr.params.get("cov_infl_mul", 1.0)), dtype=torch.float64, device="cuda")
-    return dep, logvar, alpha, rho
+    S = lambda x: torch.tensor(float(x), dtype=torch.float64, device="cuda")
+    return (dep, logvar, S(pr.params["alpha_rtpp"]), S(pr.params.get("cov_infl_mul", 1.0)),
+            S(pr.params["sigma_obs"]), S(pr.params["sigma_obsv"]))
+
+
+NAMES = ("dep", "logvar", "alpha", "rho", "sigma_h", "sigma_v")
 
 
 def _linear_loss(R3, R2):
@@ -69,19 +75,26 @@ def _fd_check(st, theta, loss, g, ncoord, seed, h=1e-4):
     construction, such as d(mean misfit)/d alpha, from dividing by zero)."""
     rng = np.random.default_rng(seed)
     worst = 0.0
-    gmax = max(float(x.abs().max()) for x in g)
-    for which in range(4):
+    gmax = max(float(x.abs().max()) for x in g[:4])
+    for which in range(len(theta)):
         n = theta[which].numel()
         coords = [()] if theta[which].dim() == 0 else [(int(j),) for j in rng.choice(n, min(ncoord, n), False)]
+        # the localisation scales are metres: a step of 1e-5 of their size,
+        # their gradient compared with itself (it has its own units)
+        loc = which >= 4
+        hw = 1e-5 * float(theta[which]) if loc else h
         for c in coords:
             tp = [t.clone() for t in theta]
             tm = [t.clone() for t in theta]
-            tp[which][c] += h
-            tm[which][c] -= h
-            fd = (_value(st, tp, loss) - _value(st, tm, loss)) / (2 * h)
+            tp[which][c] += hw
+            tm[which][c] -= hw
+            fd = (_value(st, tp, loss) - _value(st, tm, loss)) / (2 * hw)
             ad = float(g[which][c])
-            scale = max(abs(ad), 1e-3 * gmax, 1e-12)
-            worst = max(worst, abs(fd - ad) / scale)
+            scale = max(abs(ad), 1e-12) if loc else max(abs(ad), 1e-3 * gmax, 1e-12)
+            err = abs(fd - ad) / scale
+            if which >= 2:
+                _say(f"    d/d{NAMES[which]}: adjoint {ad:.6e} fd {fd:.6e} rel {err:.1e}")
+            worst = max(worst, err)
     return worst
 
 
@@ -99,16 +112,20 @@ def test_adjoint_matches_finite_differences(K, nv2d):
             L, g = _grads(st, theta, loss)
             worst = _fd_check(st, theta, loss, g, ncoord=12, seed=K)
             _say(f"  {name}: L={L:.6e} |g| dep {float(g[0].abs().max()):.3e} logvar "
-                 f"{float(g[1].abs().max()):.3e} alpha {float(g[2]):.3e} rho {float(g[3]):.3e}; "
-                 f"worst FD rel err {worst:.2e}")
+                 f"{float(g[1].abs().max()):.3e} alpha {float(g[2]):.3e} rho {float(g[3]):.3e} "
+                 f"sigma_h {float(g[4]):.3e} sigma_v {float(g[5]):.3e}; worst FD rel err {worst:.2e}")
             assert worst <= 1e-4
     finally:
         st.close()
 
 
-def _reference_loss(pn, opr, local, dep, logvar, alpha, rho, R3, R2):
+def _reference_loss(pn, opr, local, dep, logvar, alpha, rho, sh, sv, R3, R2):
     """Dense FP64 restatement: letkf_core (common_letkf.f90:61-210) with
-    torch.linalg.eigh, Analysis_Ensemble and RTPP (common_da.f90:407-461)."""
+    torch.linalg.eigh, Analysis_Ensemble and RTPP (common_da.f90:407-461);
+    rloc = exp(-((dist/sh)^2 + (dlev/sv)^2)/2) (common_da.f90:282-287) on
+    the oracle's local sets, with dist^2 and dlev^2 of each pair recovered
+    from the oracle's rloc at the problem's own scales and the vertical
+    distance rule of the search (SSH: |depth| below the surface, 0 at it)."""
     import pyoracle as po  # noqa: F401  (the local sets come from the oracle search)
     nij1, nlev, K, nv3 = pn.fcst3d.shape
     nv2 = pn.fcst2d.shape[2] if pn.fcst2d.ndim == 3 else 0
@@ -130,7 +147,15 @@ def _reference_loss(pn, opr, local, dep, logvar, alpha, rho, R3, R2):
             if len(idx):
                 ix = torch.as_tensor(idx.astype(np.int64))
                 Y = Yall[ix]
-                q = torch.as_tensor(rl) / torch.exp(logvar[ix])
+                sh0, sv0 = float(pn.params["sigma_obs"]), float(pn.params["sigma_obsv"])
+                rdep = float(pn.depth1[i, k])
+                el = pn.obselm[idx]
+                dlev = np.where(el == pn.params["id_z_obs"], abs(rdep) if k < nlev - 1 else 0.0,
+                                np.abs(pn.obslev[idx] - rdep))
+                dlev2 = torch.as_tensor(dlev ** 2)
+                dist2 = torch.as_tensor((-2.0 * np.log(rl) - dlev ** 2 / sv0 ** 2) * sh0 ** 2)
+                rloc = torch.exp(-0.5 * (dist2 / sh ** 2 + dlev2 / sv ** 2))
+                q = rloc / torch.exp(logvar[ix])
                 C = (K - 1) / rho * eye + Y.T @ (q[:, None] * Y)
                 b = Y.T @ (q * dep[ix])
                 lam, V = torch.linalg.eigh(C)
@@ -174,9 +199,29 @@ def test_adjoint_matches_fp64_reference_gradient():
         Lr.backward()
         _say(f"  L gpu {L:.12e} ref {float(Lr):.12e}")
         assert abs(L - float(Lr)) <= 1e-10 * max(1.0, abs(float(Lr)))
-        for name, ga, gr in zip(("dep", "logvar", "alpha", "rho"), g, [t.grad for t in th]):
+        for name, ga, gr in zip(NAMES, g, [t.grad for t in th]):
             err = float((ga.cpu() - gr).abs().max()) / max(float(gr.abs().max()), 1e-300)
             _say(f"  d/d{name}: max rel err {err:.2e}")
             assert err <= 1e-8
+    finally:
+        st.close()
+
+
+def test_adjoint_is_bit_reproducible():
+    """Two adjoint passes of the same analysis give the same bits (the
+    per-obs sums run in point order after a stable sort, the scalar sums in
+    a fixed block order)."""
+    _say("case adjoint determinism")
+    pr = _problem(K=32, nobs=900, seed=13)
+    st = ag.AnalysisState(pr)
+    try:
+        theta = _params(pr)
+        gen = torch.Generator(device="cuda").manual_seed(3)
+        R3 = torch.randn(pr.fcst3d.shape, generator=gen, device="cuda", dtype=torch.float64)
+        R2 = torch.randn(pr.fcst2d.shape, generator=gen, device="cuda", dtype=torch.float64)
+        _, g1 = _grads(st, theta, _linear_loss(R3, R2))
+        _, g2 = _grads(st, theta, _linear_loss(R3, R2))
+        for name, a, b in zip(NAMES, g1, g2):
+            assert torch.equal(a, b), name
     finally:
         st.close()

@@ -13,7 +13,8 @@
       nobsl equal there.
   c3  256x256x32, K=32, 5e4 obs: the adjoint of the squared misfit of the
       analysis mean to the synthetic truth against central differences
-      (h = 1e-4, FP64) on 20 coordinates, <= 1e-4 relative.
+      (h = 1e-4, FP64) on 22 coordinates (obs values and log variances,
+      alpha, rho and the localisation scales), <= 1e-4 relative.
 The reference's analysis is common_da.f90:22-124.
 """
 import numpy as np
@@ -126,17 +127,17 @@ def test_c3_full_size_adjoint_vs_finite_differences():
 
     st = ag.AnalysisState(pr, "fp64", "plain")
     try:
-        theta = [pr.obsdep.clone(), torch.log(pr.obserr.clone() ** 2),
-                 torch.tensor(float(pr.params["alpha_rtpp"]), dtype=torch.float64, device="cuda"),
-                 torch.tensor(float(pr.params.get("cov_infl_mul", 1.0)), dtype=torch.float64, device="cuda")]
+        S = lambda x: torch.tensor(float(x), dtype=torch.float64, device="cuda")
+        theta = [pr.obsdep.clone(), torch.log(pr.obserr.clone() ** 2), S(pr.params["alpha_rtpp"]),
+                 S(pr.params.get("cov_infl_mul", 1.0)), S(pr.params["sigma_obs"]), S(pr.params["sigma_obsv"])]
         th = [t.clone().requires_grad_(True) for t in theta]
         a3, a2 = ag.analysis(st, *th)
         L = loss(a3, a2)
         L.backward()
         g = [t.grad.detach() for t in th]
-        gmax = max(float(x.abs().max()) for x in g)
+        gmax = max(float(x.abs().max()) for x in g[:4])
         _say(f"  L={float(L):.6e} |g| dep {float(g[0].abs().max()):.3e} logvar {float(g[1].abs().max()):.3e} "
-             f"alpha {float(g[2]):.3e} rho {float(g[3]):.3e}")
+             f"alpha {float(g[2]):.3e} rho {float(g[3]):.3e} sigma_h {float(g[4]):.3e} sigma_v {float(g[5]):.3e}")
 
         def value(t):
             with torch.no_grad():
@@ -149,10 +150,12 @@ def test_c3_full_size_adjoint_vs_finite_differences():
         picks = [(0, j) for j in big + [int(j) for j in rng.choice(m["nobs"], 5, False)]]
         big = [int(j) for j in torch.topk(g[1].abs(), 4).indices.cpu()]
         picks += [(1, j) for j in big + [int(j) for j in rng.choice(m["nobs"], 5, False)]]
-        picks += [(2, None), (3, None)]
-        h = 1e-4
+        picks += [(2, None), (3, None), (4, None), (5, None)]
         worst = 0.0
         for which, j in picks:
+            # the localisation scales (metres): a step of 1e-5 of their size,
+            # the gradient compared with itself (its own units)
+            h = 1e-5 * float(theta[which]) if which >= 4 else 1e-4
             tp = [t.clone() for t in theta]
             tm = [t.clone() for t in theta]
             if j is None:
@@ -164,10 +167,10 @@ def test_c3_full_size_adjoint_vs_finite_differences():
                 tm[which][j] -= h
                 ad = float(g[which][j])
             fd = (value(tp) - value(tm)) / (2 * h)
-            err = abs(fd - ad) / max(abs(ad), 1e-3 * gmax)
+            err = abs(fd - ad) / (max(abs(ad), 1e-12) if which >= 4 else max(abs(ad), 1e-3 * gmax))
             worst = max(worst, err)
-            _say(f"  d/d{('dep', 'logvar', 'alpha', 'rho')[which]}[{j}]: adjoint {ad:.6e} fd {fd:.6e} "
-                 f"rel {err:.1e}")
+            _say(f"  d/d{('dep', 'logvar', 'alpha', 'rho', 'sigma_h', 'sigma_v')[which]}[{j}]: adjoint {ad:.6e} "
+                 f"fd {fd:.6e} rel {err:.1e}")
         assert worst <= 1e-4
     finally:
         st.close()
