@@ -38,8 +38,8 @@ constexpr int kMaxVars = 8;
 // FP64 copy of obshdxf.  enabled == 0 for the plain form.
 struct DualRows {
   int enabled;
-  // non-null: update only points without local obs and points whose slot
-  // count is -1 (the others were updated by the fused eigensolve)
+  // non-null: update only the points whose slot count is -1 (the others
+  // were updated by the fused eigensolve or by k_noobs)
   const int32_t* fallback_count;
   const int64_t* loff;
   long long lbase;
@@ -54,9 +54,80 @@ struct DualRows {
 // What one launch of k_transform updates.
 enum TransformMode {
   TM_ALL = 0,    // every point (the LDS eigensolve path)
-  TM_NOOBS = 1,  // points without local obs only (letkf_core_noobs)
   TM_OVF = 2     // points whose eigensolve log overflowed (E4 decomposition)
 };
+
+// Points without local obs on the register path (letkf_core_noobs: W =
+// sqrt(rho) I, w = 0, common_letkf.f90:37-57), then RTPP / RTPS /
+// round_off: one thread per (point, variable), the members walked in order
+// (consecutive threads hold consecutive points, so every member row is one
+// coalesced run); the passes after the first re-read the point's members
+// from the caches.  Points with local obs are left to the eigensolve.
+template <typename T>
+__global__ void __launch_bounds__(256) k_noobs(KParams kp, Tables<T> tb, long long p0, long long p1,
+                                               const int32_t* __restrict__ nloc, const T* __restrict__ X3,
+                                               const T* __restrict__ X2, T* __restrict__ A3, T* __restrict__ A2) {
+  const long long p = p0 + blockIdx.x * 256LL + threadIdx.x;
+  const int v = blockIdx.y;
+  if (p >= p1 || nloc[p] != 0) return;
+  const int K = kp.K, nv3 = kp.nv3d;
+  const long long P = kp.P, surf0 = (long long)kp.nij1 * (kp.nlev - 1);
+  const T* x;
+  T* y;
+  long long st;  // member stride
+  if (v < nv3) {
+    x = X3 + p + P * (long long)K * v;
+    y = A3 + p + P * (long long)K * v;
+    st = P;
+  } else {
+    if (p < surf0) return;
+    x = X2 + (p - surf0) + (long long)kp.nij1 * K * (v - nv3);
+    y = A2 + (p - surf0) + (long long)kp.nij1 * K * (v - nv3);
+    st = kp.nij1;
+  }
+  const double rK = 1.0 / (double)K;
+  const double sqrt_rho = (double)sqrt((T)kp.rho);
+  const double ap = (double)(T)kp.alpha_rtpp, as_ = (double)(T)kp.alpha_rtps;
+  // member loops unrolled by 8: eight loads in flight per thread
+  double s = 0.0;
+#pragma unroll 8
+  for (int m = 0; m < K; ++m) s += (double)x[m * st];
+  const double fm = s * rK;
+  double sd = 0.0, sa = 0.0;
+#pragma unroll 8
+  for (int m = 0; m < K; ++m) {
+    const double d = (double)x[m * st] - fm;
+    sd = fma(d, d, sd);
+    sa += fm + d * sqrt_rho;
+  }
+  const double fs = sqrt(sd / (double)(K - 1));
+  const double am = sa * rK;
+  double sda = 0.0;
+  if (as_ > 0.0)
+#pragma unroll 8
+    for (int m = 0; m < K; ++m) {
+      const double da = fm + ((double)x[m * st] - fm) * sqrt_rho - am;
+      sda = fma(da, da, sda);
+    }
+  const double asd = sqrt(sda / (double)(K - 1));
+  const int k = (int)(p / kp.nij1), i = (int)(p - (long long)k * kp.nij1);
+  const bool clampable = kp.roff && k >= 1 && !((T)tb.phi1[i] == (T)1) && v < nv3;
+  const bool is_t = v == kp.iv3d_t - 1, is_s = v == kp.iv3d_s - 1;
+  const double lo = is_t ? (double)(T)kp.tmin : (double)(T)kp.smin;
+  const double hi = is_t ? (double)(T)kp.tmax : (double)(T)kp.smax;
+  const double f = asd == 0.0 ? 0.0 : as_ * fs / asd + (1.0 - as_);
+#pragma unroll 8
+  for (int m = 0; m < K; ++m) {
+    const double d = (double)x[m * st] - fm;
+    const double a = fm + d * sqrt_rho;
+    const double da = a - am;
+    double out = a;
+    if (ap > 0.0) out = am + ap * d + (1.0 - ap) * da;
+    if (as_ > 0.0) out = am + da * f;
+    if (clampable && (is_t || is_s)) out = fmin(fmax(out, lo), hi);
+    y[m * st] = (T)out;
+  }
+}
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, long long p0, long long p1, int TI,
@@ -87,7 +158,7 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
     bool need = t < ntile;
     if (need && MODE != TM_ALL) {
       const long long p = pt + t;
-      need = MODE == TM_NOOBS ? nloc[p] == 0 : nloc[p] > 0 && dr.fallback_count[actidx[p] - a0] == -1;
+      need = nloc[p] > 0 && dr.fallback_count[actidx[p] - a0] == -1;
     }
     need_s[t] = need;
   }
@@ -136,43 +207,6 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
     const int nvp = surf ? nvt : nv3;
     if (!need_s[tp]) continue;  // updated by the fused eigensolve
     const bool clampable = kp.roff && k >= 1 && !((T)tb.phi1[i] == (T)1);
-    if (MODE == TM_NOOBS) {
-      // letkf_core_noobs: W = sqrt(rho) I, w = 0, then RTPP / RTPS /
-      // round_off as below, one variable at a time
-      for (int v = 0; v < nvp; ++v) {
-        const double x0 = h0 ? xs[((size_t)v * K + m0) * TI + tp] : 0.0;
-        const double x1 = h1 ? xs[((size_t)v * K + m1) * TI + tp] : 0.0;
-        const double fmv = wsum(x0 + x1) * rK;
-        const double d0 = h0 ? x0 - fmv : 0.0, d1 = h1 ? x1 - fmv : 0.0;
-        const double fsv = sqrt(wsum(d0 * d0 + d1 * d1) / (double)(K - 1));
-        const double a0v = fmv + d0 * sqrt_rho, a1v = fmv + d1 * sqrt_rho;
-        const double am = wsum(h0 * a0v + h1 * a1v) * rK;
-        const double da0 = h0 ? a0v - am : 0.0, da1 = h1 ? a1v - am : 0.0;
-        const double asd = sqrt(wsum(da0 * da0 + da1 * da1) / (double)(K - 1));
-        double y0 = a0v, y1 = a1v;
-        if (ap > 0.0) {
-          y0 = am + ap * d0 + (1.0 - ap) * da0;
-          y1 = am + ap * d1 + (1.0 - ap) * da1;
-        }
-        if (as_ > 0.0) {
-          const double f = asd == 0.0 ? 0.0 : as_ * fsv / asd + (1.0 - as_);
-          y0 = am + da0 * f;
-          y1 = am + da1 * f;
-        }
-        if (clampable && v < nv3) {
-          if (v == kp.iv3d_t - 1) {
-            y0 = fmin(fmax(y0, tmin), tmax);
-            y1 = fmin(fmax(y1, tmin), tmax);
-          } else if (v == kp.iv3d_s - 1) {
-            y0 = fmin(fmax(y0, smin), smax);
-            y1 = fmin(fmax(y1, smin), smax);
-          }
-        }
-        if (h0) xs[((size_t)v * K + m0) * TI + tp] = y0;
-        if (h1) xs[((size_t)v * K + m1) * TI + tp] = y1;
-      }
-      continue;
-    }
     const int n = nloc[p];
     // per-variable forecast mean and perturbations (members m0, m1)
     double dx0[kMaxVars], dx1[kMaxVars], fm[kMaxVars], fs[kMaxVars], ta0[kMaxVars], ta1[kMaxVars];
@@ -184,7 +218,7 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
       dx1[v] = h1 ? x1 - fm[v] : 0.0;
       fs[v] = sqrt(wsum(dx0[v] * dx0[v] + dx1[v] * dx1[v]) / (double)(K - 1));
     }
-    if (MODE == TM_NOOBS || n == 0) {
+    if (n == 0) {
       _Pragma("unroll") for (int v = 0; v < kMaxVars; ++v) if (v < nvp) {
         ta0[v] = fm[v] + dx0[v] * sqrt_rho;
         ta1[v] = fm[v] + dx1[v] * sqrt_rho;
@@ -403,7 +437,7 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
       if (h0) xs[((size_t)v * K + m0) * TI + tp] = y0;
       if (h1) xs[((size_t)v * K + m1) * TI + tp] = y1;
       // NaN / Inf in obsdep, obshdxf or the forecast of a point with local obs
-      if (MODE != TM_NOOBS && !isfinite(wsum((h0 ? y0 : 0.0) + (h1 ? y1 : 0.0))) && lane == 0)
+      if (!isfinite(wsum((h0 ? y0 : 0.0) + (h1 ? y1 : 0.0))) && lane == 0)
         atomicOr(dr.status, ST_NONFINITE);
     }
   }
@@ -454,15 +488,10 @@ hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0
     return hipGetLastError();
   };
   if (!fallback_count) return big(k_transform<T, TM_ALL>);
-  // register path: the points without local obs on the lean variant (LDS
-  // holds the tile only), then the overflowed eigensolves, if any
-  if (tile > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_transform<T, TM_NOOBS>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL((k_transform<T, TM_NOOBS>), dim3((unsigned)blocks), dim3(256), tile, s, kp, tb, p0, p1, TI,
-                     nloc, actidx, a0, lam, V, bvec, fcst3d, fcst2d, anal3d, anal2d, dr);
+  // register path: the points without local obs, then the overflowed
+  // eigensolves, if any
+  hipLaunchKernelGGL(k_noobs<T>, dim3((unsigned)((p1 - p0 + 255) / 256), (unsigned)nvt), dim3(256), 0, s, kp, tb, p0,
+                     p1, nloc, fcst3d, fcst2d, anal3d, anal2d);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return big(k_transform<T, TM_OVF>);

@@ -6,8 +6,8 @@
       oracle <= 2e-5; nobsl equal at every point.
   c2  512x512x40, K=64, 2e5 swath + profile obs, and
   c1  the same geometry, K=100, 6 variables with the tracer:
-      the chosen form (batched, FP32 Gram on MFMA, FP64 eigensolve) against
-      the plain FP64 form: analysis-mean max error <= 1e-5 relative to the
+      the batched form with the Gram on MFMA (BF16 hi/lo splits, the chosen
+      form, and FP32), FP64 eigensolve, against the plain FP64 form: analysis-mean max error <= 1e-5 relative to the
       variable's max |mean| (the target bound), members <= 1e-4; the plain
       FP64 form against the FP64 oracle <= 1e-10 on 3000 sampled points,
       nobsl equal there.
@@ -84,17 +84,18 @@ def test_full_size_chosen_form_and_fp64_sample(name):
     K, nv3, nv2 = m["nmem"], m["nv3d"], m["nv2d"]
     torch.cuda.synchronize()
     _say(f"  {m['nlon']}x{m['nlat']}x{m['nlev']} K={K} nv3d={nv3} nv2d={nv2} nobs={m['nobs']}")
-    a3c, a2c, _, stc = _run(prd, "fp32", "batched")
-    _say(f"  chosen form: {stc['ms_total']:.0f} ms, dual points {stc['dual_points']}")
     a3p, a2p, nlp, stp = _run(prd, "fp64", "plain")
     _say(f"  plain FP64: {stp['ms_total']:.0f} ms")
-    errs = mean_spread_errors(a3c, a2c, a3p, a2p, nv3, nv2, K)
-    for v, e in enumerate(errs):
-        _say(f"  var {v}: mean {e['mean']:.2e} spread {e['sprd']:.2e} members {e['member']:.2e}")
-    del a3c, a2c
-    torch.cuda.empty_cache()
-    assert max(e["mean"] for e in errs) <= 1e-5
-    assert max(e["member"] for e in errs) <= 1e-4
+    for prec in ("bf16x3", "fp32"):
+        a3c, a2c, _, stc = _run(prd, prec, "batched")
+        _say(f"  batched {prec}: {stc['ms_total']:.0f} ms, dual points {stc['dual_points']}")
+        errs = mean_spread_errors(a3c, a2c, a3p, a2p, nv3, nv2, K)
+        for v, e in enumerate(errs):
+            _say(f"    var {v}: mean {e['mean']:.2e} spread {e['sprd']:.2e} members {e['member']:.2e}")
+        del a3c, a2c
+        torch.cuda.empty_cache()
+        assert max(e["mean"] for e in errs) <= 1e-5
+        assert max(e["member"] for e in errs) <= 1e-4
     # the plain FP64 form against the FP64 oracle on sampled points
     rng = np.random.default_rng(20261204)
     P = m["nlon"] * m["nlat"] * m["nlev"]
