@@ -30,6 +30,8 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp32p": 157.3, "fp32x3": 157.3, "fp32d": 157.3, "bf16x3": 2500.0}   # dense, per operand type of the Gram
 PEAK_HBM_GBS = 8000.0
 EIG_KINDS = ["k_eigr_tridiag", "k_eigr_ql", "k_eigr_qvec", "k_eigr_vrot", "k_eigr_vout"]
+# form "batched": kind 3 holds the rational functions' shifted solves (no QL, no replay)
+EIG_KINDS_RAT = ["k_eigr_tridiag", "k_eigr_ql", "k_eigr_qvec", "k_eigr_rat", "k_eigr_vout"]
 
 
 def log(*a):
@@ -46,7 +48,7 @@ def parse():
                     choices=["fp64", "fp32", "fp32p", "fp32x3", "fp32d", "bf16x3"],
                     help="Gram arithmetic of the timed form (default: BF16 hi/lo splits on MFMA with FP32 "
                          "accumulation, as the K=64 configuration names)")
-    ap.add_argument("--form", default="batched", choices=["plain", "batched"])
+    ap.add_argument("--form", default="batched", choices=["plain", "batched", "batched_ql"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -749,7 +751,8 @@ def run_form(prd, precision, form, steps, warmup, autotune=False):
     st = eng.stats()
     eng.close()
     stage = {k: v / steps for k, v in tot.items()}
-    stage["eig_kernels"] = {EIG_KINDS[q // 5] + f"<{(16, 32, 48, 64, 128)[q % 5]}>": {"ms": kms[q] / steps,
+    kinds = EIG_KINDS_RAT if form == "batched" else EIG_KINDS
+    stage["eig_kernels"] = {kinds[q // 5] + f"<{(16, 32, 48, 64, 128)[q % 5]}>": {"ms": kms[q] / steps,
                                                                                     "launches": kn[q] / steps}
                             for q in range(25) if kn[q]}
     stage["autotune"] = tune
@@ -1075,6 +1078,16 @@ def main():
                 "stage_ms": {k: v for k, v in bstage.items() if k.startswith("ms_")}}
             log(f"  {result_extra[f'{alt}_batched']}")
             del b3, b2
+        if ref3 is not None and args.form == "batched":
+            log("timing the batched form with the QL eigen-decomposition of T (batched_ql) beside it")
+            q3, q2, qsec, qstage, _ = run_form(prd, args.precision, "batched_ql", 2, 1)
+            result_extra["batched_ql"] = {
+                "value": P / qsec, "ms_per_step": qsec * 1e3,
+                "speedup_vs_plain_fp64": plain_sec / qsec if plain_sec else None,
+                "errors_vs_fp64": field_errors(q3, q2, ref3, ref2, nv3, nv2, K)["summary"],
+                "stage_ms": {k: v for k, v in qstage.items() if k.startswith("ms_")}}
+            log(f"  {result_extra['batched_ql']}")
+            del q3, q2
         if not args.no_cpu:
             log("CPU: the oracle's letkf_core next to the reference's own (REAL(4), nmem 128)")
             result_extra["cpu_port_vs_reference_core"] = core_calibration()

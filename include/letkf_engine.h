@@ -57,9 +57,16 @@ typedef enum letkf_precision {
 typedef enum letkf_form {
   LETKF_FORM_PLAIN = 0,     /* nmem x nmem eigensolve at every point with */
                             /* local obs, as letkf_core does             */
-  LETKF_FORM_BATCHED = 1    /* points binned by local-obs count; points  */
+  LETKF_FORM_BATCHED = 1,   /* points binned by local-obs count; points  */
                             /* with nobsl < nmem solve the equivalent    */
-                            /* nobsl x nobsl problem                     */
+                            /* nobsl x nobsl problem; (nmem <= 128) the  */
+                            /* inverse and inverse square root of the    */
+                            /* tridiagonalised matrix from shifted       */
+                            /* solves (Zolotarev's rational function,    */
+                            /* relative error < 1e-15) instead of its    */
+                            /* eigen-decomposition                       */
+  LETKF_FORM_BATCHED_QL = 2 /* as BATCHED, with the QL eigen-            */
+                            /* decomposition of the tridiagonal matrix   */
 } letkf_form_t;
 
 typedef struct letkf_params {
@@ -132,15 +139,18 @@ typedef struct letkf_stats {
   int64_t n_eigk[25];      /* the eigensolve kernels and their launch     */
                            /* counts, index 5*kind + bucket; kind 0      */
                            /* tridiagonalisation, 1 QL, 2 Q^T on the     */
-                           /* vectors, 3 rotation replay, 4 Q and the    */
-                           /* update; bucket 0..4 = size <= 16, 32, 48,  */
-                           /* 64, 128                                    */
+                           /* vectors, 3 rotation replay (form BATCHED: */
+                           /* the rational functions' shifted solves),  */
+                           /* 4 Q and the update; bucket 0..4 = size    */
+                           /* <= 16, 32, 48, 64, 128                    */
   int64_t rotations;       /* stage_timing 2, K <= 128: QL rotations     */
                            /* logged over all slots (each replayed on   */
                            /* 1+nv3d+nv2d vectors, twice)               */
   int64_t fallback_slots;  /* K <= 128: points whose rotation log       */
-                           /* overflowed, solved by the LDS eigensolver */
-                           /* and the full tile transform instead       */
+                           /* overflowed (form BATCHED: whose matrix is */
+                           /* not SPD or whose spectrum bound is past   */
+                           /* the rational table), solved by the LDS    */
+                           /* eigensolver and the full tile transform   */
 } letkf_stats_t;
 
 typedef struct letkf_handle letkf_handle_t;
@@ -508,6 +518,17 @@ int letkf_get_stats(const letkf_handle_t* h, letkf_stats_t* out);
  * exposes it for diagnostics and for checking the local-obs selection.
  * Runs on `stream` (NULL: the handle's) and returns when done. */
 int letkf_get_nobsl(letkf_handle_t* h, int32_t* nobsl, void* stream);
+
+/* One entry of the coefficient table of the batched form's matrix
+ * functions (LETKF_FORM_BATCHED): Zolotarev's rational approximation
+ * x^-1/2 ~ d + sum_l a[l] / (x + s[l]) on [1, 2^(cls+1)], kind 0 (the
+ * nmem x nmem form, max relative error < 1e-15) or 1 (the nobsl x nobsl
+ * form, < 1e-16).  Writes d and the poles' a, s (up to 40 each) and
+ * returns their count (0: no entry; the class's matrices go to the
+ * fallback), or a negative status.  No counterpart in the reference (it
+ * uses the eigen-decomposition, common_letkf.f90:132-208); exposed for
+ * tests and for callers that check the approximation. */
+int letkf_rational_table_entry(int32_t kind, int32_t cls, double* d, double* a, double* s);
 const char* letkf_last_error(void);
 
 #ifdef __cplusplus

@@ -17,7 +17,7 @@ STATUS_NAMES = {0: "LETKF_OK", -1: "LETKF_ERR_ARG", -2: "LETKF_ERR_HIP",
                 -5: "LETKF_ERR_NONFINITE"}
 
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP32_PANEL, PREC_FP32X3, PREC_FP32_DUAL64 = 0, 1, 2, 3, 4, 5
-FORM_PLAIN, FORM_BATCHED = 0, 1
+FORM_PLAIN, FORM_BATCHED, FORM_BATCHED_QL = 0, 1, 2
 
 # every symbol include/letkf_engine.h declares
 EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
@@ -28,7 +28,7 @@ EXPORTS = ("letkf_params_default", "letkf_create", "letkf_destroy",
            "letkf_da_main_status", "letkf_autotune", "letkf_set_tuning", "letkf_get_tuning",
            "letkf_ensemble_mesp_device", "letkf_grid_pack_device", "letkf_lpfgm_core_batched",
            "letkf_da_main_lpfgm_device", "letkf_da_main_lpfgm", "letkf_get_nobsl",
-           "letkf_core_batched_noalloc")
+           "letkf_core_batched_noalloc", "letkf_rational_table_entry")
 
 
 class LpfgmCfg(C.Structure):
@@ -140,6 +140,7 @@ def lib():
                                             C.POINTER(C.c_int32)] + [vp] * 11 + [C.POINTER(ObsSet)])
         L.letkf_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.letkf_get_nobsl.argtypes = [vp, vp, vp]
+        L.letkf_rational_table_entry.argtypes = [C.c_int32, C.c_int32, vp, vp, vp]
         L.letkf_core_batched_noalloc.argtypes = [C.POINTER(Params), C.c_int32] + [vp] * 9
         i32 = C.c_int32
         L.letkf_ensemble_mesp_device.argtypes = [i32] * 6 + [C.c_double] + [vp] * 7

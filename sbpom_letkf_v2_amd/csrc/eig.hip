@@ -13,7 +13,10 @@
 // O(n^2) work of every Householder step and every QL rotation is spread
 // over the 64 lanes (lane r owns rows r, r+64); the scalar QL recurrence is
 // evaluated in every lane.
+#include <cmath>
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 #include <hipcub/hipcub.hpp>
 

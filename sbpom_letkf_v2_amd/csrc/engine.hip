@@ -50,7 +50,7 @@ static int check_params(const letkf_params_t* p) {
       p->precision != LETKF_PREC_FP32_PANEL && p->precision != LETKF_PREC_FP32X3 &&
       p->precision != LETKF_PREC_FP32_DUAL64)
     return set_err(LETKF_ERR_ARG, "unknown precision %d", p->precision);
-  if (p->form != LETKF_FORM_PLAIN && p->form != LETKF_FORM_BATCHED)
+  if (p->form != LETKF_FORM_PLAIN && p->form != LETKF_FORM_BATCHED && p->form != LETKF_FORM_BATCHED_QL)
     return set_err(LETKF_ERR_ARG, "unknown form %d", p->form);
   if ((long long)p->nij1 * p->nlev > (1LL << 31) - 2)
     return set_err(LETKF_ERR_ARG, "nij1*nlev beyond int32 range");
@@ -133,6 +133,20 @@ extern "C" int letkf_get_nobsl(letkf_handle_t* h, int32_t* nobsl, void* stream) 
     HIP_TRY(hipMemcpyAsync(nobsl, h->nloc.ptr, bytes, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
   return LETKF_OK;
+}
+
+extern "C" int letkf_rational_table_entry(int32_t kind, int32_t cls, double* d, double* a, double* s) {
+  if (!d || !a || !s) return set_err(LETKF_ERR_ARG, "NULL argument");
+  if (kind < 0 || kind > 1 || cls < 0 || cls >= kRatTableClasses)
+    return set_err(LETKF_ERR_ARG, "kind must be 0 or 1 and cls in [0,%d)", kRatTableClasses);
+  const double* e = rat_table_data() + ((size_t)kind * kRatTableClasses + cls) * (2 + 2 * kRatTablePoles);
+  const int n = (int)e[0];
+  *d = e[1];
+  for (int l = 0; l < n; ++l) {
+    a[l] = e[2 + l];
+    s[l] = e[2 + kRatTablePoles + l];
+  }
+  return n;
 }
 
 extern "C" int letkf_get_stats(const letkf_handle_t* h, letkf_stats_t* out) {
@@ -384,7 +398,9 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   HIP_TRY(h->act_p.ensure(sizeof(int32_t) * (P + 1)));
   HIP_TRY(h->scan_tmp.ensure(scan_tmp_bytes(P)));
   HIP_TRY(h->yT.ensure(sizeof(double) * (size_t)o->nobs * K));
-  const int dual = p.form == LETKF_FORM_BATCHED;
+  const int dual = p.form != LETKF_FORM_PLAIN;
+  // batched form, register path: f(T) by rational functions (no QL log)
+  const bool rat = p.form == LETKF_FORM_BATCHED && K <= 128;
   // count_nobsl for every point, then the scans
   if (timing) {
     HIP_TRY(h->ensure_events(2));
@@ -407,8 +423,9 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   long long PC;
   if (K <= 128) {
     const double ncol = 1.0 + p.nv3d + p.nv2d;
-    const double per_slot = 16.0 * K * K + 8.0 * K * ncol + 16.0 * eigr_rot_capacity(K) +
-                            (double)eigr_ix_pitch(K) + 8.0 * eigr_sweep_capacity(K) + 64.0 * K + 8.0;
+    const double log_bytes =
+        rat ? 0.0 : 16.0 * eigr_rot_capacity(K) + (double)eigr_ix_pitch(K) + 8.0 * eigr_sweep_capacity(K);
+    const double per_slot = 16.0 * K * K + 8.0 * K * ncol + log_bytes + 64.0 * K + 8.0;
     size_t mfree = 0, mtotal = 0;
     HIP_TRY(hipMemGetInfo(&mfree, &mtotal));
     const double held = (double)(h->G.bytes + h->V.bytes + h->yvec.bytes + h->lam.bytes + h->bvec.bytes +
@@ -444,7 +461,7 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   HIP_TRY(hipMemcpyAsync(bv.data(), d_out, sizeof(long long) * (2 * (nch + 1) + 6), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   st.max_nobsl = bv[2 * (nch + 1)];
-  st.dual_points = p.form == LETKF_FORM_BATCHED ? bv[2 * (nch + 1) + 1] : 0;
+  st.dual_points = dual ? bv[2 * (nch + 1) + 1] : 0;
   long long maxL = 0, maxA = 0;
   for (long long c = 0; c < nch; ++c) {
     maxL = std::max(maxL, bv[2 * (c + 1)] - bv[2 * c]);
@@ -497,13 +514,21 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   const long long hcap = reg_eig ? 2 * eigr_sweep_capacity(K) : eig_log_capacity(K); // int32 (col,len)
   HIP_TRY(h->eD.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
   HIP_TRY(h->eE.ensure(sizeof(double) * (size_t)(maxA + 1) * K));
-  HIP_TRY(h->logc.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
-  if (!reg_eig) HIP_TRY(h->logs.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
-  HIP_TRY(h->logi.ensure(sizeof(int32_t) * (size_t)(maxA + 1) * hcap));
-  HIP_TRY(h->logn.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
-  if (reg_eig) {
-    HIP_TRY(h->logx.ensure((size_t)(maxA + 1) * eigr_ix_pitch(K)));
-    HIP_TRY(h->logr.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
+  HIP_TRY(h->logn.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));  // per slot: sweeps logged, or -1 fallback
+  if (!rat) {  // the QL rotation log
+    HIP_TRY(h->logc.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
+    if (!reg_eig) HIP_TRY(h->logs.ensure(sizeof(double) * (size_t)(maxA + 1) * cap));
+    HIP_TRY(h->logi.ensure(sizeof(int32_t) * (size_t)(maxA + 1) * hcap));
+    if (reg_eig) {
+      HIP_TRY(h->logx.ensure((size_t)(maxA + 1) * eigr_ix_pitch(K)));
+      HIP_TRY(h->logr.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
+    }
+  }
+  if (rat && !h->ratt_ok) {  // the coefficient table, once per handle (host copy built once per process)
+    HIP_TRY(h->ratt.ensure(sizeof(double) * rat_table_size()));
+    HIP_TRY(hipMemcpyAsync(h->ratt.ptr, rat_table_data(), sizeof(double) * rat_table_size(),
+                           hipMemcpyHostToDevice, s));
+    h->ratt_ok = true;
   }
   st.chunks = nch;
   st.gram_flops = 2.0 * K * ((double)bv[2 * (nch + 1) + 2] + (double)K * (double)bv[2 * (nch + 1) + 3]);
@@ -523,6 +548,7 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
   pl.yp = yp;
   pl.reg_eig = reg_eig;
   pl.dual = dual;
+  pl.rat = rat;
   pl.st = st;
   pl.valid = true;
   pl.epoch = ++h->plans_made;
@@ -592,7 +618,8 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
                                         h->lam.as<double>(), h->V.as<double>(), h->yvec.as<double>(), yp, kp, tb, f3,
                                         f2, a3, a2, h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
                                         h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
-                                        h->status.as<int>(), h->vpath[q], ktm, s));
+                                        h->status.as<int>(), h->vpath[q],
+                                        pl.rat ? h->ratt.as<double>() : nullptr, ktm, s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -990,7 +1017,8 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
                                           h->V.as<double>(), h->yvec.as<double>(), pl.yp, kp, tb, f3, f2, a3, a2,
                                           h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
                                           h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
-                                          h->status.as<int>(), v, nullptr, s));
+                                          h->status.as<int>(), v, pl.rat ? h->ratt.as<double>() : nullptr,
+                                          nullptr, s));
             HIP_TRY(hipEventRecord(e1, s));
             HIP_TRY(hipEventSynchronize(e1));
             float t = 0.f;

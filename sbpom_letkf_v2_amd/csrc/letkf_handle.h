@@ -58,6 +58,7 @@ struct AnalysisPlan {
   long long P = 0, PC = 0, nch = 0, yp = 0;
   bool reg_eig = false;
   int dual = 0;
+  bool rat = false;  // f(T) by rational functions (batched form, K <= 128)
   std::vector<long long> bpts, bv;  // chunk point bounds; per-chunk (local obs, active slot) offsets + totals
   std::vector<int32_t> hist, boff;  // per (chunk, size bucket) slot counts and list offsets
   letkf_stats_t st{};               // counters known from the plan
@@ -88,6 +89,8 @@ struct letkf_handle {
   DevBuf eD, eE, logc, logs, logi, logn, logx, logr, yvec;  // eigensolve workspaces
   DevBuf rotsum;  // stage_timing 2: logged QL rotations, summed over slots
   DevBuf otrig;   // (sin, cos) of every obs latitude
+  DevBuf ratt;    // coefficient table of the rational path (uploaded once: ratt_ok)
+  bool ratt_ok = false;
   DevBuf adj_part;  // adjoint: per-point (dL/dalpha, dL/drho, dL/dsigma_obs, dL/dsigma_obsv), block sums
   DevBuf adj_pair, adj_sort;  // adjoint: per-pair terms; sort keys, values and scratch
   DevBuf bhist, bcur, blist;              // per-chunk size buckets of active slots
@@ -117,8 +120,9 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &ratt, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
+    ratt_ok = false;
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();
     for (hipEvent_t e : kevents) (void)hipEventDestroy(e);

@@ -164,6 +164,10 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
 // (count[slot] == -1, and status[1] set) for launch_transform's fallback
 // mode.  vpath selects the vector path (0 default rule, 1 Q accumulated
 // from the reflectors, K <= 64 only; 2 reflectors applied to the vectors).
+// rat (the table of rat_table_data on the device): f(T) on the vectors by
+// rational functions and shifted tridiagonal solves instead of E2 + V2;
+// slots it cannot take (not SPD, spectrum past the table) go to the same
+// fallback.
 // Events around every eigensolve kernel launch (stage_timing 2): launch u
 // is bracketed by ev[2u], ev[2u+1] and tagged 4*kind + bucket.
 struct EigTimer {
@@ -188,7 +192,17 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const int64_t* loff, long long lbase,
                               const int32_t* lobs, const double* lrloc,
                               const double* yT, const double* bvec,
-                              int* status, int vpath, EigTimer* tm, hipStream_t s);
+                              int* status, int vpath, const double* rat, EigTimer* tm,
+                              hipStream_t s);
+// The rational path's coefficient table (ratfun.inc), rat_table_size()
+// doubles, built once per process; rat == nullptr above selects E2 + V2.
+// Layout: [kind][class][n, d, a[kRatTablePoles], s[kRatTablePoles]].
+constexpr int kRatTableClasses = 24, kRatTablePoles = 40;
+const double* rat_table_data();
+size_t rat_table_size();
+// Zolotarev's approximation of x^-1/2 on [1, kappa] of order n: d, a[n],
+// s[n]; returns its max relative error.
+long double rat_invsqrt(long double kappa, int n, long double* d, long double* a, long double* s);
 
 // Adjoint of the analysis (FP64, K <= 64): the active slots [0,na) of a
 // chunk (C and b from launch_gram_fp64 in the K x K form) and the points

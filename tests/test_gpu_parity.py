@@ -99,23 +99,68 @@ def test_nobs_zero_copies_forecast():
     assert torch.equal(a3, prd.fcst3d) and torch.equal(a2, prd.fcst2d)
 
 
+@pytest.mark.parametrize("form", ["batched", "batched_ql"])
 @pytest.mark.parametrize("K,nobs,seed", [(20, 300, 51), (20, 100, 54), (64, 700, 52),
                                          (100, 1800, 53), (128, 2000, 55)])
-def test_batched_form_matches_fp64_oracle(K, nobs, seed):
-    """Batched form: points with nobsl < K solve the nobsl x nobsl problem.
-    Same LETKF, so the FP64 tolerance is unchanged (1e-10).  A 1-cell /
-    1-level localisation scale gives a mix of nobsl < K and nobsl >= K."""
-    _say(f"case batched K={K} nobs={nobs} seed={seed}")
+def test_batched_form_matches_fp64_oracle(K, nobs, seed, form):
+    """Batched form: points with nobsl < K solve the nobsl x nobsl problem;
+    "batched" applies the inverse and inverse square root of the
+    tridiagonalised matrix by shifted solves (Zolotarev's rational function,
+    relative error < 1e-15), "batched_ql" through its QL eigen-
+    decomposition.  Same LETKF, so the FP64 tolerance is unchanged (1e-10).
+    A 1-cell / 1-level localisation scale gives a mix of nobsl < K and
+    nobsl >= K."""
+    _say(f"case {form} K={K} nobs={nobs} seed={seed}")
     pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=K, nobs=nobs, seed=seed,
                             alpha_rtpp=0.7, alpha_rtps=0.3, land_frac=0.1, real_kind=8,
                             sigma_h_cells=1.0, sigma_v=1.0)
-    a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched")
+    a3, a2, a3o, a2o, nl, st = _run_both(pr, form=form)
     act = nl[nl > 0]
-    _say(f"  n<K points {(act < K).sum()} n>=K points {(act >= K).sum()}")
+    _say(f"  n<K points {(act < K).sum()} n>=K points {(act >= K).sum()} fallback {st['fallback_slots']}")
     assert (act < K).sum() > 0 and st["local_obs"] == int(nl.sum())
+    assert st["fallback_slots"] == 0
     e3 = per_var_err(a3, a3o, axis_var=3)
     e2 = per_var_err(a2, a2o, axis_var=2)
     _say(f"  err3={e3} err2={e2}")
+    assert max(e3) <= 1e-10 and max(e2) <= 1e-10
+
+
+@pytest.mark.parametrize("K,scale", [(32, 0.01), (64, 0.01), (100, 0.01), (64, 0.003)])
+def test_batched_rational_ill_conditioned(K, scale):
+    """Obs errors shrunk by `scale`: the local matrices' spectra widen to
+    kappa ~ 1e4-1e7 (up to ~36 poles of the rational function); the FP64
+    batched form stays within 1e-10 of the oracle, with no fallback at
+    scale 0.01; at 0.003 the widest spectra pass the table's last class
+    (2^24) and those slots take the fallback within the same analysis."""
+    _say(f"case batched ill-conditioned K={K} obs error x{scale}")
+    pr = synth.make_problem(nlon=20, nlat=18, nlev=5, nmem=K, nobs=1500, seed=60 + K,
+                            alpha_rtpp=0.5, land_frac=0.1, real_kind=8)
+    pr.obserr = pr.obserr * scale
+    a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched")
+    e3 = per_var_err(a3, a3o, axis_var=3)
+    e2 = per_var_err(a2, a2o, axis_var=2)
+    _say(f"  maxn={nl.max()} fallback {st['fallback_slots']} err3={max(e3):.2e} err2={max(e2):.2e}")
+    if scale >= 0.01:
+        assert st["fallback_slots"] == 0
+    else:
+        assert 0 < st["fallback_slots"] < st["active_points"]
+    assert max(e3) <= 1e-10 and max(e2) <= 1e-10
+
+
+@pytest.mark.parametrize("K", [48, 100])
+def test_batched_rational_fallback(K, monkeypatch):
+    """LETKF_RAT_CLASSES=0 (a test hook) removes every class of the rational
+    table, so every active slot takes the fallback: the LDS eigensolve and
+    the tile update with the reference's eigenvalue rule."""
+    _say(f"case batched forced fallback K={K}")
+    monkeypatch.setenv("LETKF_RAT_CLASSES", "0")
+    pr = synth.make_problem(nlon=16, nlat=14, nlev=4, nmem=K, nobs=700, seed=70 + K,
+                            alpha_rtpp=0.7, alpha_rtps=0.2, real_kind=8, sigma_h_cells=1.5)
+    a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched")
+    e3 = per_var_err(a3, a3o, axis_var=3)
+    e2 = per_var_err(a2, a2o, axis_var=2)
+    _say(f"  active {st['active_points']} fallback {st['fallback_slots']} err3={max(e3):.2e} err2={max(e2):.2e}")
+    assert st["fallback_slots"] == st["active_points"] > 0
     assert max(e3) <= 1e-10 and max(e2) <= 1e-10
 
 
@@ -227,7 +272,7 @@ def test_rotation_log_overflow_fallback(monkeypatch):
     _say("case rotation-log overflow fallback K=32")
     pr = synth.make_problem(nlon=24, nlat=20, nlev=6, nmem=32, nobs=800, seed=71, alpha_rtpp=0.7,
                             alpha_rtps=0.3, land_frac=0.1, real_kind=8, sigma_h_cells=1.0, sigma_v=1.0)
-    a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched")
+    a3, a2, a3o, a2o, nl, st = _run_both(pr, form="batched_ql")
     _say(f"  fallback slots {st['fallback_slots']} of {st['active_points']}")
     assert 0 < st["fallback_slots"] < st["active_points"]
     e3 = per_var_err(a3, a3o, axis_var=3)

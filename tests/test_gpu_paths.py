@@ -5,7 +5,7 @@ FP64 oracle (FP64: <= 1e-10 of the field scale; REAL(4): <= 2e-5):
     past tmin/tmax/smin/smax at random points, with land points
     (phi1 == 1) and the bottom level k = 1 that the clamp skips; through
     the register eigensolve's update, the update of points without local
-    obs, the overflow fallback (LETKF_ROT_LOG_CAP), the 128 bucket, the
+    obs, the fallback (LETKF_ROT_LOG_CAP, LETKF_RAT_CLASSES), the 128 bucket, the
     LPF core and REAL(4) I/O;
   * the tracer layout (5 3-D variables + SSH: 7 vector columns) and the
     widest one (7 + 1 variables: 9 columns) through both vector paths of
@@ -115,16 +115,20 @@ def test_round_off_fires(K, nobs, form, real_kind):
                 assert np.array_equal(a3[:, :, :, v][at], a3o[:, :, :, v][at])
 
 
-def test_round_off_fires_overflow_fallback(monkeypatch):
-    """Slots whose QL log overflows are updated by the full tile transform."""
-    monkeypatch.setenv("LETKF_ROT_LOG_CAP", "300")
-    _say("case round_off overflow fallback K=32")
+@pytest.mark.parametrize("form,hook", [("batched_ql", ("LETKF_ROT_LOG_CAP", "300")),
+                                       ("batched", ("LETKF_RAT_CLASSES", "1"))])
+def test_round_off_fires_overflow_fallback(monkeypatch, form, hook):
+    """Slots whose QL log overflows (batched_ql), or whose spectrum bound
+    lies past the rational table's classes in use (batched: one class,
+    kappa <= 2), are updated by the full tile transform."""
+    monkeypatch.setenv(*hook)
+    _say(f"case round_off fallback K=32 {form} {hook}")
     pr = synth.make_problem(nlon=20, nlat=16, nlev=5, nmem=32, nobs=600, seed=91, alpha_rtpp=0.6,
                             alpha_rtps=0.3, land_frac=0.2, real_kind=8, sigma_h_cells=1.5, sigma_v=1.0)
     pr = _push_out_of_bounds(pr, 6)
     a3o, a2o, nl = _oracle(pr)
     _check_clamps(a3o, pr)
-    a3, a2, st = _engine(pr, "fp64", "batched")
+    a3, a2, st = _engine(pr, "fp64", form)
     _say(f"  fallback slots {st['fallback_slots']} of {st['active_points']}")
     assert st["fallback_slots"] > 0
     e3 = per_var_err(a3, a3o, axis_var=3)
@@ -208,7 +212,7 @@ def _unaffected_points_agree(pr_clean, j, a3):
     assert err <= 1e-10
 
 
-@pytest.mark.parametrize("form", ["batched", "plain"])
+@pytest.mark.parametrize("form", ["batched", "batched_ql", "plain"])
 @pytest.mark.parametrize("bad", ["nan_dep", "inf_hdxf", "zero_err"])
 def test_nonfinite_inputs_are_reported(bad, form):
     _say(f"case failure path {bad} {form}")
