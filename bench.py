@@ -525,7 +525,8 @@ def run_c4(args):
     throughput = all cycles' points / all cycles' analysis time (chosen
     form), per-cycle error of the chosen form against the FP64 engine
     (batched, FP64 Gram) on the same cycle.  Graph: every cycle planned on
-    its own handle (letkf_da_main_plan, workspaces capped), the eight
+    its own handle (letkf_da_main_plan, workspaces capped at an equal share
+    of the free memory), the eight
     enqueues captured in ONE hipGraph on one stream and replayed; a replay
     reproduces every cycle's eager analysis bit for bit (checksums of the
     output bits)."""
@@ -555,11 +556,14 @@ def run_c4(args):
     if not args.no_graph:
         log("c4: every cycle planned on its own handle, the eight enqueues captured in one graph")
         engs = []
+        # the eight handles hold their workspaces together: each gets an equal
+        # share of three quarters of the free device memory
+        share = int(0.75 * torch.cuda.mem_get_info()[0] / len(probs))
         try:
             t0 = time.perf_counter()
             for pr, a3, a2, _, _ in probs:
                 prm = engine.make_params(pr, args.precision, args.form)
-                prm.workspace_bytes = 8 << 30
+                prm.workspace_bytes = share
                 eng = engine.Engine(prm)
                 eng.plan_device(pr)
                 engs.append(eng)
