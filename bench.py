@@ -7,12 +7,17 @@ ensemble generated on the GPU.  A step is one full analysis of every grid
 point, inputs resident in HBM.
 
 Forms:
-  plain    FP64 throughout, K x K eigensolve at every point with local obs
-  batched  points with nobsl < K solved in the nobsl x nobsl form; with
-           --precision fp32 / bf16x3 the Gram products run on MFMA (FP64
-           eigensolve).  Default: batched + bf16x3 (BF16 hi/lo splits, FP32
-           accumulation), timed next to plain FP64; the FP32 Gram is timed
-           beside it.
+  plain       FP64 throughout, K x K eigen-decomposition at every point with
+              local obs
+  batched     points with nobsl < K solved in the nobsl x nobsl form, the
+              matrix functions from rational approximations and shifted
+              tridiagonal solves; with --precision fp32 / bf16x3 the Gram
+              products run on MFMA (the rest FP64), bf16x3t also runs the
+              Householder reduction in FP32
+  batched_ql  the batched form with the QL eigen-decomposition
+Default: batched + bf16x3t (BF16 hi/lo splits with FP32 accumulation for the
+Gram, FP32 Householder reduction), timed next to plain FP64; bf16x3 (FP64
+reduction) and the batched_ql form are timed beside it.
 
 Prints one JSON line (rank 0).  Progress goes to stderr.
 """
@@ -27,7 +32,8 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 
 METRIC = "analysed grid pts/s at K=64: plain FP64 vs batched mixed-prec; max err vs FP64"
 PEAK_FP64_TFLOPS = 78.6
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp32p": 157.3, "fp32x3": 157.3, "fp32d": 157.3, "bf16x3": 2500.0}   # dense, per operand type of the Gram
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp32p": 157.3, "fp32x3": 157.3, "fp32d": 157.3, "bf16x3": 2500.0,
+               "bf16x3t": 2500.0}   # dense, per operand type of the Gram
 PEAK_HBM_GBS = 8000.0
 EIG_KINDS = ["k_eigr_tridiag", "k_eigr_ql", "k_eigr_qvec", "k_eigr_vrot", "k_eigr_vout"]
 # form "batched": kind 3 holds the rational functions' shifted solves (no QL, no replay)
@@ -44,10 +50,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="bf16x3",
-                    choices=["fp64", "fp32", "fp32p", "fp32x3", "fp32d", "bf16x3"],
+    ap.add_argument("--precision", default="bf16x3t",
+                    choices=["fp64", "fp32", "fp32p", "fp32x3", "fp32d", "bf16x3", "bf16x3t"],
                     help="Gram arithmetic of the timed form (default: BF16 hi/lo splits on MFMA with FP32 "
-                         "accumulation, as the K=64 configuration names)")
+                         "accumulation, as the K=64 configuration names, and the Householder reduction in FP32)")
     ap.add_argument("--form", default="batched", choices=["plain", "batched", "batched_ql"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle baseline sample")
@@ -1021,9 +1027,11 @@ def main():
         gram_kernel: dict(ms=stage["ms_gram"], launches=nch, work=st["gram_flops"], unit="TFLOP/s",
                           bound="valu" if gram_kernel == "k_gram_fp64" else "mfma",
                           peak=PEAK_TFLOPS[args.precision]),
-        # Householder tridiagonalisation: (4/3) m^3 flops per point, FP64 on the vector ALUs
+        # Householder tridiagonalisation: (4/3) m^3 flops per point on the vector ALUs, FP64 (FP32 for
+        # bf16x3t in the batched form: 157.3 TFLOP/s)
         "k_eigr_tridiag": dict(ms=tri_ms, launches=tri_n, work=st["eig_flops"], unit="TFLOP/s",
-                               bound="valu", peak=PEAK_FP64_TFLOPS),
+                               bound="valu", peak=157.3 if (args.precision == "bf16x3t" and args.form == "batched")
+                               else PEAK_FP64_TFLOPS),
         # implicit QL: ~30 FP64 flops per logged rotation (Givens pair, refined rsqrt, shift update)
         "k_eigr_ql": dict(ms=ql_ms, launches=ql_n, work=30.0 * rot, unit="TFLOP/s",
                           bound="valu", peak=PEAK_FP64_TFLOPS),
@@ -1084,9 +1092,11 @@ def main():
             del b3, b2
         if ref3 is not None and args.form == "batched":
             log("timing the batched form with the QL eigen-decomposition of T (batched_ql) beside it")
-            q3, q2, qsec, qstage, _ = run_form(prd, args.precision, "batched_ql", 2, 1)
+            # (the FP32 reduction of bf16x3t belongs to the rational path: the QL form runs it in FP64)
+            qprec = "bf16x3" if args.precision == "bf16x3t" else args.precision
+            q3, q2, qsec, qstage, _ = run_form(prd, qprec, "batched_ql", 2, 1)
             result_extra["batched_ql"] = {
-                "value": P / qsec, "ms_per_step": qsec * 1e3,
+                "precision": qprec, "value": P / qsec, "ms_per_step": qsec * 1e3,
                 "speedup_vs_plain_fp64": plain_sec / qsec if plain_sec else None,
                 "errors_vs_fp64": field_errors(q3, q2, ref3, ref2, nv3, nv2, K)["summary"],
                 "stage_ms": {k: v for k, v in qstage.items() if k.startswith("ms_")}}
@@ -1100,7 +1110,7 @@ def main():
         "metric": METRIC, "value": P / sec, "unit": "grid points/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else f"{args.precision}+f64",
+        "dtype": {"fp64": "f64", "bf16x3t": "bf16x3+f32+f64"}.get(args.precision, f"{args.precision}+f64"),
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {meta['nlon']}x{meta['nlat']}x{meta['nlev']} "
                                f"K={K} nv3d={nv3} nv2d={nv2} nobs={meta['nobs']} "

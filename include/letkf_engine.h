@@ -49,8 +49,11 @@ typedef enum letkf_precision {
   LETKF_PREC_FP32X3 = 4,    /* Gram from FP32 hi/lo splits of the FP64   */
                             /* operands, hi*hi + hi*lo + lo*hi on FP32   */
                             /* MFMA with FP32 accumulation, FP64 eig     */
-  LETKF_PREC_FP32_DUAL64 = 5 /* FP32 MFMA Gram for the K x K form, FP64  */
+  LETKF_PREC_FP32_DUAL64 = 5,/* FP32 MFMA Gram for the K x K form, FP64  */
                             /* Gram for the n x n dual form (batched)    */
+  LETKF_PREC_BF16X3_TRI32 = 6 /* as BF16X3, the Householder reduction of */
+                            /* the Gram (nmem <= 128) in FP32; the      */
+                            /* matrix functions and the update in FP64  */
 } letkf_precision_t;
 
 /* How the per-point ensemble-space problem is solved. */

@@ -17,6 +17,7 @@ STATUS_NAMES = {0: "LETKF_OK", -1: "LETKF_ERR_ARG", -2: "LETKF_ERR_HIP",
                 -5: "LETKF_ERR_NONFINITE"}
 
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP32_PANEL, PREC_FP32X3, PREC_FP32_DUAL64 = 0, 1, 2, 3, 4, 5
+PREC_BF16X3_TRI32 = 6
 FORM_PLAIN, FORM_BATCHED, FORM_BATCHED_QL = 0, 1, 2
 
 # every symbol include/letkf_engine.h declares

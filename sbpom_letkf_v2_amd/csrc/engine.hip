@@ -48,7 +48,7 @@ static int check_params(const letkf_params_t* p) {
   if (p->workspace_bytes < 0) return set_err(LETKF_ERR_ARG, "workspace_bytes must be >= 0");
   if (p->precision != LETKF_PREC_FP64 && p->precision != LETKF_PREC_FP32 && p->precision != LETKF_PREC_BF16X3 &&
       p->precision != LETKF_PREC_FP32_PANEL && p->precision != LETKF_PREC_FP32X3 &&
-      p->precision != LETKF_PREC_FP32_DUAL64)
+      p->precision != LETKF_PREC_FP32_DUAL64 && p->precision != LETKF_PREC_BF16X3_TRI32)
     return set_err(LETKF_ERR_ARG, "unknown precision %d", p->precision);
   if (p->form != LETKF_FORM_PLAIN && p->form != LETKF_FORM_BATCHED && p->form != LETKF_FORM_BATCHED_QL)
     return set_err(LETKF_ERR_ARG, "unknown form %d", p->form);
@@ -329,6 +329,7 @@ static hipError_t launch_gram_any(int prec, const KParams& kp, const Tables<T>& 
                                   const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
                                   const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
                                   hipStream_t s) {
+  if (prec == LETKF_PREC_BF16X3_TRI32) prec = LETKF_PREC_BF16X3;  // the Gram of the mode
   if (prec == LETKF_PREC_FP64 || kp.K > 128)
     return launch_gram_fp64<T>(kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec, dual, s);
   if (prec != LETKF_PREC_FP32_DUAL64 || !dual)
@@ -638,7 +639,8 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
                                         f2, a3, a2, h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
                                         h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
                                         h->status.as<int>(), h->vpath[q],
-                                        pl.rat ? h->ratt.as<double>() : nullptr, ktm, s));
+                                        pl.rat ? h->ratt.as<double>() : nullptr,
+                                        p.precision == LETKF_PREC_BF16X3_TRI32, ktm, s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -1037,7 +1039,7 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
                                           h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
                                           h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
                                           h->status.as<int>(), v, pl.rat ? h->ratt.as<double>() : nullptr,
-                                          nullptr, s));
+                                          p.precision == LETKF_PREC_BF16X3_TRI32, nullptr, s));
             HIP_TRY(hipEventRecord(e1, s));
             HIP_TRY(hipEventSynchronize(e1));
             float t = 0.f;

@@ -164,6 +164,7 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
 // (count[slot] == -1, and status[1] set) for launch_transform's fallback
 // mode.  vpath selects the vector path (0 default rule, 1 Q accumulated
 // from the reflectors, K <= 64 only; 2 reflectors applied to the vectors).
+// tri32: E1 (the Householder reduction) in FP32 arithmetic.
 // rat (the table of rat_table_data on the device): f(T) on the vectors by
 // rational functions and shifted tridiagonal solves instead of E2 + V2;
 // slots it cannot take (not SPD, spectrum past the table) go to the same
@@ -192,7 +193,7 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const int64_t* loff, long long lbase,
                               const int32_t* lobs, const double* lrloc,
                               const double* yT, const double* bvec,
-                              int* status, int vpath, const double* rat, EigTimer* tm,
+                              int* status, int vpath, const double* rat, bool tri32, EigTimer* tm,
                               hipStream_t s);
 // The rational path's coefficient table (ratfun.inc), rat_table_size()
 // doubles, built once per process; rat == nullptr above selects E2 + V2.

@@ -50,4 +50,37 @@ __device__ __forceinline__ void wave_sum2(double& a, double& b) {
   b = __longlong_as_double(((long long)yh << 32) | (unsigned)yl);
 }
 
+// The same sums for floats (one DPP move per step).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_f<0xB1, 0xF>(v);
+  v += dpp_f<0x4E, 0xF>(v);
+  v += dpp_f<0x141, 0xF>(v);
+  v += dpp_f<0x140, 0xF>(v);
+  v += dpp_f<0x142, 0xA>(v);
+  v += dpp_f<0x143, 0xC>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+__device__ __forceinline__ void wave_sum2(float& a, float& b) {
+  a += dpp_f<0xB1, 0xF>(a);
+  b += dpp_f<0xB1, 0xF>(b);
+  a += dpp_f<0x4E, 0xF>(a);
+  b += dpp_f<0x4E, 0xF>(b);
+  a += dpp_f<0x141, 0xF>(a);
+  b += dpp_f<0x141, 0xF>(b);
+  a += dpp_f<0x140, 0xF>(a);
+  b += dpp_f<0x140, 0xF>(b);
+  a += dpp_f<0x142, 0xA>(a);
+  b += dpp_f<0x142, 0xA>(b);
+  a += dpp_f<0x143, 0xC>(a);
+  b += dpp_f<0x143, 0xC>(b);
+  a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
+  b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
+}
+
 }  // namespace letkf

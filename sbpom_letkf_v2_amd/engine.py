@@ -15,11 +15,11 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (FORM_BATCHED, FORM_BATCHED_QL, FORM_PLAIN, PREC_BF16X3, PREC_FP32, PREC_FP32_DUAL64,
-                   PREC_FP32_PANEL, PREC_FP32X3, PREC_FP64, check)
+from ._lib import (FORM_BATCHED, FORM_BATCHED_QL, FORM_PLAIN, PREC_BF16X3, PREC_BF16X3_TRI32, PREC_FP32,
+                   PREC_FP32_DUAL64, PREC_FP32_PANEL, PREC_FP32X3, PREC_FP64, check)
 
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp32p": PREC_FP32_PANEL,
-              "fp32x3": PREC_FP32X3, "fp32d": PREC_FP32_DUAL64}
+              "fp32x3": PREC_FP32X3, "fp32d": PREC_FP32_DUAL64, "bf16x3t": PREC_BF16X3_TRI32}
 FORMS = {"plain": FORM_PLAIN, "batched": FORM_BATCHED, "batched_ql": FORM_BATCHED_QL}
 
 

@@ -19,11 +19,12 @@ MODULE letkf_engine_mod
   PUBLIC :: letkf_params_default, letkf_create, letkf_destroy, letkf_set_params, letkf_da_main, letkf_da_main_lpfgm
   PUBLIC :: letkf_last_error_f
   PUBLIC :: LETKF_PREC_FP64, LETKF_PREC_FP32, LETKF_PREC_BF16X3, LETKF_PREC_FP32_PANEL, LETKF_PREC_FP32X3, &
-            LETKF_PREC_FP32_DUAL64
+            LETKF_PREC_FP32_DUAL64, LETKF_PREC_BF16X3_TRI32
   PUBLIC :: LETKF_FORM_PLAIN, LETKF_FORM_BATCHED, LETKF_FORM_BATCHED_QL
 
   INTEGER(c_int32_t), PARAMETER :: LETKF_PREC_FP64 = 0, LETKF_PREC_FP32 = 1, LETKF_PREC_BF16X3 = 2
   INTEGER(c_int32_t), PARAMETER :: LETKF_PREC_FP32_PANEL = 3, LETKF_PREC_FP32X3 = 4, LETKF_PREC_FP32_DUAL64 = 5
+  INTEGER(c_int32_t), PARAMETER :: LETKF_PREC_BF16X3_TRI32 = 6
   INTEGER(c_int32_t), PARAMETER :: LETKF_FORM_PLAIN = 0, LETKF_FORM_BATCHED = 1, LETKF_FORM_BATCHED_QL = 2
 
   TYPE, BIND(C) :: letkf_params_t

@@ -94,10 +94,18 @@ def test_argument_errors_are_status_codes():
     bad.precision = 9
     assert L.letkf_create(C.byref(bad), C.byref(h)) == -1
     # every named precision has a value the C header accepts
-    assert set(engine.PRECISIONS.values()) == {0, 1, 2, 3, 4, 5}
+    assert set(engine.PRECISIONS.values()) == {0, 1, 2, 3, 4, 5, 6}
     hdr = open(HEADER).read()
-    for name, v in (("LETKF_PREC_FP32_PANEL", 3), ("LETKF_PREC_FP32X3", 4), ("LETKF_PREC_FP32_DUAL64", 5)):
+    for name, v in (("LETKF_PREC_FP32_PANEL", 3), ("LETKF_PREC_FP32X3", 4), ("LETKF_PREC_FP32_DUAL64", 5),
+                    ("LETKF_PREC_BF16X3_TRI32", 6)):
         assert re.search(rf"{name} = {v}\b", hdr), name
+    # and every named form
+    assert set(engine.FORMS.values()) == {0, 1, 2}
+    for name, v in (("LETKF_FORM_PLAIN", 0), ("LETKF_FORM_BATCHED", 1), ("LETKF_FORM_BATCHED_QL", 2)):
+        assert re.search(rf"{name} = {v}\b", hdr), name
+    bad = _lib.Params.from_buffer_copy(p)
+    bad.form = 3
+    assert L.letkf_create(C.byref(bad), C.byref(h)) == -1
     # NULL handle / arrays
     g, o = _lib.Grid(), _lib.Obs()
     assert L.letkf_da_main(None, C.byref(g), C.byref(o), None, None, None, None) == -1

@@ -7,7 +7,8 @@
   c2  512x512x40, K=64, 2e5 swath + profile obs, and
   c1  the same geometry, K=100, 6 variables with the tracer:
       the batched form with the Gram on MFMA (BF16 hi/lo splits, the chosen
-      form, and FP32), FP64 eigensolve, against the plain FP64 form: analysis-mean max error <= 1e-5 relative to the
+      form, and FP32; BF16 hi/lo with the Householder reduction in FP32),
+      the rest FP64, against the plain FP64 form: analysis-mean max error <= 1e-5 relative to the
       variable's max |mean| (the target bound), members <= 1e-4; the plain
       FP64 form against the FP64 oracle <= 1e-10 on 3000 sampled points,
       nobsl equal there.
@@ -86,7 +87,7 @@ def test_full_size_chosen_form_and_fp64_sample(name):
     _say(f"  {m['nlon']}x{m['nlat']}x{m['nlev']} K={K} nv3d={nv3} nv2d={nv2} nobs={m['nobs']}")
     a3p, a2p, nlp, stp = _run(prd, "fp64", "plain")
     _say(f"  plain FP64: {stp['ms_total']:.0f} ms")
-    for prec in ("bf16x3", "fp32"):
+    for prec in ("bf16x3", "fp32", "bf16x3t"):
         a3c, a2c, _, stc = _run(prd, prec, "batched")
         _say(f"  batched {prec}: {stc['ms_total']:.0f} ms, dual points {stc['dual_points']}")
         errs = mean_spread_errors(a3c, a2c, a3p, a2p, nv3, nv2, K)

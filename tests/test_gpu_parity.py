@@ -286,9 +286,12 @@ def test_rotation_log_overflow_fallback(monkeypatch):
                                                    ("bf16x3", "batched", 128, 2000), ("fp32", "plain", 100, 1800),
                                                    ("fp32p", "batched", 48, 900), ("fp32p", "plain", 100, 1800),
                                                    ("fp32x3", "batched", 48, 900), ("fp32x3", "batched", 100, 1800),
-                                                   ("fp32d", "batched", 48, 900), ("fp32d", "batched", 100, 1800)])
+                                                   ("fp32d", "batched", 48, 900), ("fp32d", "batched", 100, 1800),
+                                                   ("bf16x3t", "batched", 48, 900), ("bf16x3t", "batched", 100, 1800),
+                                                   ("bf16x3t", "batched", 128, 2000)])
 def test_mixed_precision_gram_close_to_fp64(precision, form, K, nobs):
     """Gram on MFMA (FP32, FP32 summed per panel, FP32 or BF16 hi/lo splits) with FP64 eigensolve
+    (bf16x3t: the Householder reduction in FP32)
     against the FP64 oracle: the analysis-mean max error <= 1e-5 relative to
     the variable's max |mean| (the target bound of the mixed-precision
     form), every member within 1e-4 of the field scale (FP32 products keep
