@@ -735,7 +735,7 @@ def run_form(prd, precision, form, steps, warmup, autotune=False):
     if autotune:
         t = time.perf_counter()
         _, _, times, choice = eng.autotune(prd, a3, a2)
-        tune = {"seconds": time.perf_counter() - t, "bucket_ms_Q_vs_reflectors": times, "paths": choice}
+        tune = {"seconds": time.perf_counter() - t, "bucket_ms_per_path": times, "paths": choice}
         log(f"  autotune: {tune}")
     for w in range(warmup):
         t = time.perf_counter()

@@ -222,18 +222,21 @@ int letkf_da_main_status(letkf_handle_t* h, void* stream);
 
 /* In-project autotuner (no counterpart in the reference).  The register
  * eigensolve groups the active points of a chunk by local-problem size into
- * buckets of at most 16, 32, 48, 64 and 128; for nmem <= 64 each bucket has
- * two vector paths: 1 = the orthogonal factor Q of the tridiagonalisation
- * accumulated and applied as a matrix, 2 = its Householder reflectors
- * applied to the vectors one by one.  letkf_autotune times both on the
- * first chunk of the given analysis (DEVICE arrays, as
- * letkf_da_main_device), keeps the faster per bucket in the handle (the
- * fixed rule's path unless the other is more than 3% faster), then
- * runs the whole analysis with them (anal3d/anal2d receive it).
- * ms (may be NULL) receives 10 times: ms[2q + v - 1] for bucket q (16, 32,
+ * buckets of at most 16, 32, 48, 64 and 128; each bucket has up to three
+ * vector paths: 1 = the orthogonal factor Q of the tridiagonalisation
+ * accumulated and applied as a matrix (nmem <= 64), 2 = its Householder
+ * reflectors applied to the vectors one wavefront sum per vector, 3 = the
+ * reflectors applied to all the vectors at once in a (row group, column)
+ * lane layout (nv3d + nv2d <= 7).  letkf_autotune times every path a bucket
+ * can take on the first chunk of the given analysis (DEVICE arrays, as
+ * letkf_da_main_device), keeps the fastest per bucket in the handle (the
+ * fixed rule's path unless another is more than 3% faster), then runs the
+ * whole analysis with them (anal3d/anal2d receive it).
+ * ms (may be NULL) receives 15 times: ms[3q + v - 1] for bucket q (16, 32,
  * 48, 64, 128) under path v, -1 where not measured.  letkf_set_tuning /
- * letkf_get_tuning set or read the five choices (0 = the fixed rule:
- * reflectors for buckets 64 and 128 or nmem > 64, Q below). */
+ * letkf_get_tuning set or read the five choices (0 = the fixed rule: path
+ * 3 where nv3d + nv2d <= 7, else 2 for buckets 64 and 128 or nmem > 64 and
+ * 1 below; a path a bucket cannot take falls back to the rule). */
 int letkf_autotune(letkf_handle_t* h, const letkf_grid_t* grid,
                    const letkf_obs_t* obs, const void* fcst3d,
                    const void* fcst2d, void* anal3d, void* anal2d, double* ms,

@@ -988,20 +988,22 @@ extern "C" int letkf_da_main_lpfgm_device(letkf_handle_t* h, const letkf_grid_t*
 }
 
 // In-project autotuner of the register eigensolve's vector path: on the
-// first chunk of this analysis, every size bucket (16..64, K <= 64) runs its
-// eigensolve + update with the Q path and with the reflector path, three
-// timed runs each after a warm-up, and keeps the faster; then the whole
-// analysis runs with the chosen paths.  The fixed rule's path is kept
-// unless the other one is faster by more than 3% (timing noise on a tie
-// must not flip it).  ms[2q + v - 1] receives the best time of bucket q
-// under path v (-1: not measured).
+// first chunk of this analysis, every size bucket runs its eigensolve +
+// update under each vector path it can take (1: Q accumulated, K <= 64;
+// 2: reflectors, one wavefront sum per column; 3: reflectors in the
+// blocked layout, 1 + nv <= 8), three timed runs each after a warm-up, and
+// keeps the fastest; then the whole analysis runs with the chosen paths.
+// The fixed rule's path is kept unless another is faster by more than 3%
+// (timing noise on a tie must not flip it).  ms[3q + v - 1] receives the
+// best time of bucket q under path v (-1: not measured).
 template <typename T>
 static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t* o, const T* f3, const T* f2,
                            T* a3, T* a2, hipStream_t s, double* ms) {
-  for (int q = 0; q < 2 * kNQ; ++q) ms[q] = -1.0;
+  for (int q = 0; q < 3 * kNQ; ++q) ms[q] = -1.0;
   const letkf_params_t& p = h->prm;
   const int K = p.nmem;
-  if (K <= 64 && o->nobs > 0) {
+  const int ncol = 1 + p.nv3d + p.nv2d;
+  if (K <= 128 && o->nobs > 0) {
     int rc = plan_device<T>(h, g, o, s, false);
     if (rc) return rc;
     const AnalysisPlan& pl = h->plan;
@@ -1023,10 +1025,11 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
       hipEvent_t e0, e1;
       HIP_TRY(hipEventCreate(&e0));
       HIP_TRY(hipEventCreate(&e1));
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kNQ; ++q) {
         const int cnt = pl.hist[q];
         if (cnt == 0) continue;
-        for (int v = 1; v <= 2; ++v) {
+        for (int v = 1; v <= 3; ++v) {
+          if (eigr_vpath(v, bucket_nb(q), K, ncol) != v) continue;  // a path this bucket cannot take
           float best = 1e30f;
           for (int r = 0; r < 4; ++r) {  // run 0 warms up
             HIP_TRY(hipEventRecord(e0, s));
@@ -1046,11 +1049,16 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
             HIP_TRY(hipEventElapsedTime(&t, e0, e1));
             if (r > 0 && t < best) best = t;
           }
-          ms[2 * q + v - 1] = best;
+          ms[3 * q + v - 1] = best;
         }
-        // the fixed rule's path unless the other is faster by more than 3%
-        const int dflt = bucket_nb(q) >= 64 ? 2 : 1, alt = 3 - dflt;
-        h->vpath[q] = ms[2 * q + alt - 1] < 0.97 * ms[2 * q + dflt - 1] ? alt : dflt;
+        // the fixed rule's path unless another is faster by more than 3%
+        const int dflt = eigr_vpath(0, bucket_nb(q), K, ncol);
+        int pick = dflt;
+        for (int v = 1; v <= 3; ++v)
+          if (ms[3 * q + v - 1] >= 0.0 && ms[3 * q + v - 1] < 0.97 * ms[3 * q + dflt - 1] &&
+              (pick == dflt || ms[3 * q + v - 1] < ms[3 * q + pick - 1]))
+            pick = v;
+        h->vpath[q] = pick;
       }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -1065,7 +1073,7 @@ extern "C" int letkf_autotune(letkf_handle_t* h, const letkf_grid_t* g, const le
   if (rc) return rc;
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-  double loc[2 * kNQ];
+  double loc[3 * kNQ];
   double* m = ms ? ms : loc;
   if (h->prm.real_kind == 8)
     return autotune_device<double>(h, g, o, (const double*)f3, (const double*)f2, (double*)a3, (double*)a2, s, m);
@@ -1075,7 +1083,7 @@ extern "C" int letkf_autotune(letkf_handle_t* h, const letkf_grid_t* g, const le
 extern "C" int letkf_set_tuning(letkf_handle_t* h, const int32_t* vpath) {
   if (!h || !vpath) return set_err(LETKF_ERR_ARG, "NULL argument");
   for (int q = 0; q < kNQ; ++q)
-    if (vpath[q] < 0 || vpath[q] > 2) return set_err(LETKF_ERR_ARG, "vector path %d of bucket %d not in 0..2", vpath[q], q);
+    if (vpath[q] < 0 || vpath[q] > 3) return set_err(LETKF_ERR_ARG, "vector path %d of bucket %d not in 0..3", vpath[q], q);
   for (int q = 0; q < kNQ; ++q) h->vpath[q] = vpath[q];
   return LETKF_OK;
 }

@@ -163,7 +163,10 @@ hipError_t launch_eig3(int K, int na, const double* G, double* Q, double* D,
 // alias V.  Slots whose rotation log overflowed get lam/V instead
 // (count[slot] == -1, and status[1] set) for launch_transform's fallback
 // mode.  vpath selects the vector path (0 default rule, 1 Q accumulated
-// from the reflectors, K <= 64 only; 2 reflectors applied to the vectors).
+// from the reflectors, K <= 64 only; 2 reflectors applied to the vectors
+// one wavefront sum per column; 3 reflectors applied in the (row group,
+// column) layout, 1 + nv3d + nv2d <= 8); eigr_vpath resolves it for a
+// bucket (a path the bucket cannot take falls back to the rule).
 // tri32: E1 (the Householder reduction) in FP32 arithmetic.
 // rat (the table of rat_table_data on the device): f(T) on the vectors by
 // rational functions and shifted tridiagonal solves instead of E2 + V2;
@@ -180,6 +183,7 @@ struct EigTimer {
 
 long long eigr_rot_capacity(int K);
 long long eigr_sweep_capacity(int K);
+int eigr_vpath(int vpath, int nb, int K, int ncol);
 long long eigr_ix_pitch(int K);
 template <typename T>
 hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,

@@ -208,20 +208,24 @@ class Engine:
     BUCKETS = (16, 32, 48, 64, 128)
 
     def autotune(self, pr, anal3d=None, anal2d=None, stream=None):
-        """letkf_autotune: time the two vector paths of every size bucket on
-        the first chunk of pr's analysis, keep the faster, run the analysis.
-        Returns (anal3d, anal2d, {bucket: (ms Q path, ms reflector path)},
-        chosen paths)."""
+        """letkf_autotune: time the vector paths of every size bucket on the
+        first chunk of pr's analysis, keep the fastest, run the analysis.
+        Returns (anal3d, anal2d, {bucket: {path: ms}} for the paths
+        measured (1 Q, 2 reflectors, 3 blocked reflectors), chosen paths)."""
         import torch
         g, o = self._device_structs(pr)
         if anal3d is None:
             anal3d = torch.empty_like(pr.fcst3d)
         if anal2d is None:
             anal2d = torch.empty_like(pr.fcst2d)
-        ms = (C.c_double * 10)()
+        ms = (C.c_double * 15)()
         check(_lib.lib().letkf_autotune(self.h, C.byref(g), C.byref(o), _ptr(pr.fcst3d), _ptr(pr.fcst2d),
                                         _ptr(anal3d), _ptr(anal2d), ms, C.c_void_p(stream) if stream else None))
-        times = {nb: (ms[2 * q], ms[2 * q + 1]) for q, nb in enumerate(self.BUCKETS) if ms[2 * q] >= 0}
+        times = {}
+        for q, nb in enumerate(self.BUCKETS):
+            t = {v: ms[3 * q + v - 1] for v in (1, 2, 3) if ms[3 * q + v - 1] >= 0}
+            if t:
+                times[nb] = t
         return anal3d, anal2d, times, self.get_tuning()
 
     def get_tuning(self):
@@ -230,7 +234,8 @@ class Engine:
         return dict(zip(self.BUCKETS, list(v)))
 
     def set_tuning(self, paths):
-        """paths: {bucket: 0 default rule | 1 Q | 2 reflectors} or 5 values."""
+        """paths: {bucket: 0 default rule | 1 Q | 2 reflectors | 3 blocked
+        reflectors} or 5 values."""
         vals = [paths.get(nb, 0) for nb in self.BUCKETS] if isinstance(paths, dict) else list(paths)
         check(_lib.lib().letkf_set_tuning(self.h, (C.c_int32 * 5)(*vals)))
 

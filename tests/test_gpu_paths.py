@@ -8,7 +8,7 @@ FP64 oracle (FP64: <= 1e-10 of the field scale; REAL(4): <= 2e-5):
     obs, the fallback (LETKF_ROT_LOG_CAP, LETKF_RAT_CLASSES), the 128 bucket, the
     LPF core and REAL(4) I/O;
   * the tracer layout (5 3-D variables + SSH: 7 vector columns) and the
-    widest one (7 + 1 variables: 9 columns) through both vector paths of
+    widest one (7 + 1 variables: 9 columns) through every vector path of
     the register eigensolve;
   * failure paths: NaN in obsdep, Inf in obshdxf and obserr = 0 return
     LETKF_ERR_NONFINITE with every kernel finished and the analysis of the
@@ -152,20 +152,22 @@ def test_round_off_fires_lpf():
     assert max(e3) <= 1e-10
 
 
-@pytest.mark.parametrize("K,nv3d,tracer,paths", [(100, 5, True, None),
-                                                 (48, 5, True, "Q"), (48, 5, True, "R"),
-                                                 (64, 7, True, "Q"), (64, 7, True, "R"),
+@pytest.mark.parametrize("K,nv3d,tracer,paths", [(100, 5, True, None), (100, 5, True, "B"),
+                                                 (48, 5, True, "Q"), (48, 5, True, "R"), (48, 5, True, "B"),
+                                                 (64, 7, True, "Q"), (64, 7, True, "R"), (64, 7, True, "B"),
                                                  (100, 7, True, None)])
 def test_vector_columns_tracer_and_widest(K, nv3d, tracer, paths):
     """7 vector columns (5 3-D variables with the tracer + SSH) and 9 (7 + 1),
-    K = 100 on the reflector path (the only one above nmem 64) and K <= 64
-    with each path forced ("Q": Q accumulated, "R": reflectors)."""
+    K = 100 on the fixed rule's path and on the blocked reflectors, and
+    K <= 64 with each path forced ("Q": Q accumulated, "R": reflectors one
+    wavefront sum per column, "B": the blocked layout; 9 columns exceed
+    the blocked layout's 8 and fall back to the rule)."""
     _say(f"case columns K={K} nv3d={nv3d} paths={paths}")
     pr = synth.make_problem(nlon=20, nlat=16, nlev=5, nmem=K, nv3d=nv3d, nv2d=1, tracer=tracer,
                             nobs=1500 if K == 100 else 800, seed=101 + K + nv3d, alpha_rtpp=0.7,
                             alpha_rtps=0.2, real_kind=8, sigma_h_cells=1.0, sigma_v=1.0)
     a3o, a2o, nl = _oracle(pr)
-    vp = None if paths is None else {nb: (1 if paths == "Q" else 2) for nb in (16, 32, 48, 64)}
+    vp = None if paths is None else {nb: {"Q": 1, "R": 2, "B": 3}[paths] for nb in (16, 32, 48, 64, 128)}
     a3, a2, st = _engine(pr, "fp64", "batched", paths=vp)
     act = nl[nl > 0]
     _say(f"  nobsl<K {(act < K).sum()} nobsl>=K {(act >= K).sum()}")
