@@ -154,6 +154,9 @@ typedef struct letkf_stats {
                            /* not SPD or whose spectrum bound is past   */
                            /* the rational table), solved by the LDS    */
                            /* eigensolver and the full tile transform   */
+  int64_t bucket_slots[5]; /* K <= 128: points with local obs per size  */
+                           /* bucket of the register eigensolve (size   */
+                           /* <= 16, 32, 48, 64, 128)                   */
 } letkf_stats_t;
 
 typedef struct letkf_handle letkf_handle_t;

@@ -90,7 +90,8 @@ class Stats(C.Structure):
             "ms_total", "ms_search", "ms_gram", "ms_eig", "ms_transform",
             "gram_flops", "transform_bytes")] + [("chunks", C.c_int64), ("eig_flops", C.c_double),
                                                  ("ms_eigk", C.c_double * 25), ("n_eigk", C.c_int64 * 25),
-                                                 ("rotations", C.c_int64), ("fallback_slots", C.c_int64)]
+                                                 ("rotations", C.c_int64), ("fallback_slots", C.c_int64),
+                                                 ("bucket_slots", C.c_int64 * 5)]
 
 
 class EngineError(RuntimeError):

@@ -513,6 +513,7 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     for (size_t q = 0; q < nh; ++q) {
       boff[q] = run;
       run += hist[q];
+      st.bucket_slots[q % kNQ] += hist[q];
     }
     HIP_TRY(hipMemcpyAsync(h->bcur.ptr, boff.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_bucket_scatter, dim3(2048), dim3(256), 0, s, P, h->nloc.as<int32_t>(),

@@ -90,7 +90,7 @@ class Engine:
         out = {}
         for n, _ in s._fields_:
             v = getattr(s, n)
-            out[n] = list(v) if n in ("ms_eigk", "n_eigk") else v
+            out[n] = list(v) if n in ("ms_eigk", "n_eigk", "bucket_slots") else v
         return out
 
     def nobsl(self):
