@@ -6,6 +6,8 @@
 //       wvec = Pa (Rlinv_dYf^T dep), an exact re-association of :162-184)
 // One workgroup per local problem; its rows are staged through LDS in panels
 // of 2048/KT rows.
+#include <cstdlib>
+
 #include "letkf_internal.h"
 
 namespace letkf {
@@ -437,7 +439,10 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   constexpr int LP = OUT + 1;    // odd pitch: row-wise and column-wise panel writes hit distinct banks
   constexpr int RS = 256 / OUT;  // primal: threads sharing one output column
   constexpr int kStage = kPanel * OUT / 256;  // panel elements staged per thread (4 .. 16)
-  constexpr int kBatch = OUT == 128 ? (kSplit ? 4 : 8) : 1;  // loads issued together (occupancy: 8 at OUT 64)
+  // loads issued together: four at OUT 64 (one at a time left the K x K
+  // Gram waiting a memory latency per element; 2 / 4 / 8 measured 121 / 103
+  // / 129 ms at c2, the last at occupancy 3)
+  constexpr int kBatch = OUT == 128 ? (kSplit ? 4 : 8) : 4;
   // panel: rows = summation index (obs or member), cols = output index
   __shared__ float As[kPanel][LP];
   __shared__ float Bs[kPanel][LP];
@@ -631,6 +636,90 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   }
 }
 
+// BF16x3 Gram of the dual-form points (n < K, nmem <= 64) with one
+// wavefront per point, its MFMA operands loaded straight from the local obs
+// rows -- no LDS panel, no block barrier, no idle wavefront while the (at
+// most three) 32 x 32 tiles of M = B B^T run.  M sums over the K members in
+// chunks of 16; lane (h, c) = (lane / 32, lane % 32) loads members
+// 16 k + 8 h .. + 7 of rows c and 32 + c (eight consecutive doubles of an
+// obs row), which are both the A operand of its row tile and the B operand
+// of its column tile (A = B^T), so each value is split into BF16 hi + lo
+// once.  Operand values, their splits and the MFMA sequence are those of
+// k_gram_mfma<LETKF_PREC_BF16X3, 64> (upper-triangle tiles only); the K x K
+// points are left to it (dual == 2 there).
+template <typename T>
+__global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int na, double* __restrict__ G,
+                                                   double* __restrict__ bvec) {
+  const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5;
+  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= na) return;
+  const int n = src.begin(b);
+  if (n >= K) return;  // K x K form: k_gram_mfma
+  const bool two = n > 32;  // tiles (0,0), (0,1), (1,1); else (0,0) only
+  mf_v16 acc00 = mf_v16{}, acc01 = mf_v16{}, acc11 = mf_v16{};
+  // x -> (hi, lo) in element e of two operand vectors
+  auto split = [](double x, mf_bf8& hi, mf_bf8& lo, int e) {
+    const float v = (float)x;
+    const __bf16 vh = (__bf16)v;
+    hi[e] = vh;
+    lo[e] = (__bf16)(v - (float)vh);
+  };
+  // rows c and 32 + c (obs), their sqrt weights; bvec = sw dep
+  int io0, io1 = 0;
+  double sw0, sw1 = 0.0;
+  src.brow(c < n ? c : n - 1, io0, sw0);
+  if (two) src.brow(32 + c < n ? 32 + c : n - 1, io1, sw1);
+  if (c >= n) sw0 = 0.0;
+  if (32 + c >= n) sw1 = 0.0;
+  if (h == 0) {
+    if (c < n) bvec[(long long)b * K + c] = sw0 * src.dep(c);
+    if (32 + c < n) bvec[(long long)b * K + 32 + c] = sw1 * src.dep(32 + c);
+  }
+  const double* y0 = src.yT + (long long)io0 * K;
+  const double* y1 = src.yT + (long long)io1 * K;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    mf_bf8 x0h, x0l, x1h, x1l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int m = k0 + 8 * h + e;
+      const double v0 = m < K ? sw0 * y0[m < K ? m : K - 1] : 0.0;
+      split(v0, x0h, x0l, e);
+    }
+    acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x0h, acc00, 0, 0, 0);
+    acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x0l, acc00, 0, 0, 0);
+    acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0l, x0h, acc00, 0, 0, 0);
+    if (two) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int m = k0 + 8 * h + e;
+        const double v1 = m < K ? sw1 * y1[m < K ? m : K - 1] : 0.0;
+        split(v1, x1h, x1l, e);
+      }
+      acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x1h, acc01, 0, 0, 0);
+      acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x1l, acc01, 0, 0, 0);
+      acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0l, x1h, acc01, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1h, x1h, acc11, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1h, x1l, acc11, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1l, x1h, acc11, 0, 0, 0);
+    }
+  }
+  // upper-triangle entries of the live tiles, row pitch n
+  double* Gp = G + (long long)b * K * K;
+  auto store = [&](const mf_v16& v, int tr, int tc) {
+    const int col = tc + c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = tr + (r >> 2) * 8 + h * 4 + (r & 3);
+      if (row < n && col < n && row <= col) Gp[row * n + col] = (double)v[r];
+    }
+  };
+  store(acc00, 0, 0);
+  if (two) {
+    store(acc01, 0, 32);
+    store(acc11, 32, 32);
+  }
+}
+
 template <int PREC, typename T>
 static void launch_gram_mixed_p(int K, double diag, const ObsSource<T>& src, int na, double* G, double* bvec,
                                 int dual, hipStream_t s) {
@@ -651,6 +740,11 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
   if (kp.K > 128) return hipErrorInvalidValue;
   ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
+  if (prec == LETKF_PREC_BF16X3 && kp.K <= 64 && dual == 1 && !getenv("LETKF_GRAM_BLOCK")) {
+    // dual-form points one wavefront each, then the K x K ones
+    hipLaunchKernelGGL(k_gram_wave<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
+    dual = 2;
+  }
   if (prec == LETKF_PREC_FP32)
     launch_gram_mixed_p<LETKF_PREC_FP32, T>(kp.K, diag, src, na, G, bvec, dual, s);
   else if (prec == LETKF_PREC_FP32_PANEL)
