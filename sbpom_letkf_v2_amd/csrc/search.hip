@@ -7,6 +7,8 @@
 // NINT (round half away from zero) of r_size values, distance_2p
 // (common.f90:79-101) stores cos(d) in r_size before ACOS, and rloc is
 // rounded to r_size.  T is the caller's REAL.
+#include <cstdlib>
+
 #include "letkf_internal.h"
 
 namespace letkf {
@@ -119,6 +121,178 @@ __global__ void __launch_bounds__(256) k_search_fill(KParams kp, Tables<T> tb, l
   }
 }
 
+// ---------------------------------------------------------------- column search
+// The same acceptance tests as search_point for every level of one grid
+// column at once: one wavefront per column i, its lanes over the column's
+// candidate obs -- the CSR ranges of the box's rows, flattened in the
+// reference's walk order (row j ascending, then ip) -- so the horizontal
+// test (cos d, acos: the costly part, the same at every level) runs once per
+// (column, obs) instead of once per (point, obs).  A level loop then applies
+// the vertical test of each level to the chunk's 64 candidates; a ballot and
+// its prefix count give each accepted obs its place in the point's list, so
+// a point's obs keep the reference's order and a level's entries are stored
+// contiguously.  The running count of level k lives in lane k % 64 of cnt0
+// (k < 64) or cnt1.  FILL: the obs index and rloc of every accepted obs of the
+// levels whose points lie in [p0, p1) (search_point with dist, as
+// k_search_fill); otherwise the counts of every level (as k_search_count).
+constexpr int kColMaxLev = 128;
+
+// one wavefront's LDS writes visible to its later reads (program order)
+__device__ __forceinline__ void col_wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int lane_mbcnt(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+}
+
+__device__ __forceinline__ long long readlane_ll(long long x, int l) {
+  const int lo = __builtin_amdgcn_readlane((int)x, l);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+  return ((long long)hi << 32) | (unsigned)lo;
+}
+
+template <typename T, bool FILL>
+__global__ void __launch_bounds__(256) k_search_col(KParams kp, Tables<T> tb, long long p0, long long p1,
+                                                    int32_t* __restrict__ nloc, const int64_t* __restrict__ loff,
+                                                    int32_t* __restrict__ lobs, double* __restrict__ lrloc) {
+#pragma clang fp contract(off)
+  __shared__ int rstart[4][64];        // exclusive scan of the row lengths of a group of 64 rows
+  __shared__ long long rib[4][64];     // first ip of each row
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * 4 + w;
+  if (i >= kp.nij1) return;
+  const long long nij1 = kp.nij1;
+  int kbeg = 0, kend = kp.nlev;
+  if (FILL) {  // levels k with p = i + nij1 k in [p0, p1)
+    kbeg = p0 > i ? (int)((p0 - i + nij1 - 1) / nij1) : 0;
+    kend = p1 > i ? (int)((p1 - i + nij1 - 1) / nij1) : 0;
+    kend = kend < kp.nlev ? kend : kp.nlev;
+    if (kbeg >= kend) return;
+  }
+  const T dz = (T)kp.dist_zero, dzv = (T)kp.dist_zerov;
+  const T pi = (T)kp.pi, re = (T)kp.re;
+  const T so = (T)kp.sigma_obs, sv = (T)kp.sigma_obsv;
+  const bool vloc = !(kp.sigma_obsv <= 0.0);
+  const T hx = dz / tb.dlon1[i], hy = dz / tb.dlat1[i];
+  int imin = nint_r<T>(tb.ri1[i] - hx);
+  int imax = nint_r<T>(tb.ri1[i] + hx);
+  int jmin = nint_r<T>(tb.rj1[i] - hy);
+  int jmax = nint_r<T>(tb.rj1[i] + hy);
+  imin = imin < 1 ? 1 : imin;
+  imax = imax > kp.nlon ? kp.nlon : imax;
+  jmin = jmin < 1 ? 1 : jmin;
+  jmax = jmax > kp.nlat ? kp.nlat : jmax;
+  const T lonb = tb.lon1[i];
+  const double bb = (double)(T)(pi * tb.lat1[i]) / 180.0;
+  const double sinb = sin(bb), cosb = cos(bb);
+  // per level: running count (lane k % 64) and, FILL, the chunk-relative
+  // offset of the point's list
+  int cnt0 = 0, cnt1 = 0;
+  long long base0 = 0, base1 = 0;
+  if (FILL) {
+    const long long lb = loff[p0];
+    if (lane >= kbeg && lane < kend) base0 = loff[i + nij1 * lane] - lb;
+    if (lane + 64 >= kbeg && lane + 64 < kend) base1 = loff[i + nij1 * (lane + 64)] - lb;
+  }
+  for (int jg = jmin; jg <= jmax; jg += 64) {
+    // the rows of this group: first ip and length, exclusive scan
+    const int j = jg + lane;
+    long long ib = 1;
+    int len = 0;
+    if (j <= jmax) {
+      if (imin > 1)
+        ib = (long long)tb.nobsgrd[(imin - 2) + (long long)kp.nlon * (j - 1)] + 1;
+      else if (j > 1)
+        ib = (long long)tb.nobsgrd[(kp.nlon - 1) + (long long)kp.nlon * (j - 2)] + 1;
+      const long long ie = tb.nobsgrd[(imax - 1) + (long long)kp.nlon * (j - 1)];
+      len = ie >= ib ? (int)(ie - ib + 1) : 0;
+    }
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int up = __shfl_up(incl, o);
+      if (lane >= o) incl += up;
+    }
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    rstart[w][lane] = incl - len;
+    rib[w][lane] = ib;
+    col_wsync();
+    for (int v0 = 0; v0 < total; v0 += 64) {
+      const int v = v0 + lane;
+      const bool valid = v < total;
+      // the row holding candidate v: the last row whose start is <= v
+      int r = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (rstart[w][r + step] <= v) r += step;
+      const int io = valid ? (int)(rib[w][r] + (v - rstart[w][r]) - 1) : 0;
+      // horizontal test (search_point's arithmetic)
+      bool hok = false;
+      T dist = (T)0;
+      int el = 0;
+      T ol = (T)0;
+      if (valid) {
+        el = tb.elm[io];
+        ol = tb.olev[io];
+        const double2 sca = tb.osc[io];
+        const double cc = (double)(T)(pi * (T)(lonb - tb.olon[io])) / 180.0;
+        const double prod1 = sca.x * sinb;
+        const double prod2 = sca.y * cosb;
+        T cosd = (T)(prod1 + prod2 * cos(cc));
+        if ((double)cosd > 1.0) cosd = (T)1;
+        if ((double)cosd < -1.0) cosd = (T)-1;
+        hok = !((double)cosd <= kp.cos_out);
+        if (hok && (FILL || (double)cosd < kp.cos_in)) {
+          dist = re * acos_r<T>(cosd);
+          hok = !(dz < dist);
+        }
+      }
+      if (__ballot(hok) == 0) continue;
+      for (int k = kbeg; k < kend; ++k) {
+        const T rdep = tb.depth1[i + nij1 * k];
+        const bool below_surface = k < kp.nlev - 1;
+        T dlev;
+        if (el == kp.id_z_obs && below_surface)
+          dlev = abs_r<T>(rdep);
+        else if (el != kp.id_z_obs)
+          dlev = abs_r<T>(ol - rdep);
+        else
+          dlev = (T)0;
+        const bool acc = hok && !(dzv < dlev);
+        const unsigned long long m = __ballot(acc);
+        if (m == 0) continue;
+        const int kl = k & 63;
+        if (FILL) {
+          const int ck = __builtin_amdgcn_readlane(k < 64 ? cnt0 : cnt1, kl);
+          const long long bk = readlane_ll(k < 64 ? base0 : base1, kl);
+          if (acc) {
+            const long long pos = bk + ck + lane_mbcnt(m);
+            T q = (dist / so) * (dist / so);
+            if (vloc) q = q + (dlev / sv) * (dlev / sv);
+            lobs[pos] = io;
+            lrloc[pos] = (double)(T)exp(-0.5 * (double)q);
+          }
+        }
+        const int nacc = __popcll(m);
+        if (lane == kl) {
+          if (k < 64)
+            cnt0 += nacc;
+          else
+            cnt1 += nacc;
+        }
+      }
+    }
+    col_wsync();  // the next row group overwrites rstart / rib
+  }
+  if (!FILL) {
+    if (lane < kp.nlev) nloc[i + nij1 * lane] = cnt0;
+    if (lane + 64 < kp.nlev) nloc[i + nij1 * (lane + 64)] = cnt1;
+  }
+}
+
 // (sin, cos) of every obs latitude in radians, formed as distance_2p forms
 // them (common.f90:79-101): aa = real(pi*lat, r_size)/180 in FP64.
 template <typename T>
@@ -143,6 +317,11 @@ template hipError_t launch_obs_trig<double>(int, double, const double*, double2*
 
 template <typename T>
 hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb, int32_t* nloc, hipStream_t s) {
+  if (kp.nlev <= kColMaxLev && !getenv("LETKF_SEARCH_POINT")) {
+    hipLaunchKernelGGL((k_search_col<T, false>), dim3((unsigned)((kp.nij1 + 3) / 4)), dim3(256), 0, s, kp, tb, 0LL,
+                       kp.P, nloc, nullptr, nullptr, nullptr);
+    return hipGetLastError();
+  }
   long long blocks = (kp.P + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_search_count<T>, dim3((unsigned)blocks), dim3(256), 0, s, kp, tb, nloc);
@@ -152,6 +331,12 @@ hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb, int32_t* 
 template <typename T>
 hipError_t launch_search_fill(const KParams& kp, const Tables<T>& tb, long long p0, long long p1,
                               const int64_t* loff, int32_t* lobs, double* lrloc, hipStream_t s) {
+  if (p1 <= p0) return hipSuccess;
+  if (kp.nlev <= kColMaxLev && !getenv("LETKF_SEARCH_POINT")) {
+    hipLaunchKernelGGL((k_search_col<T, true>), dim3((unsigned)((kp.nij1 + 3) / 4)), dim3(256), 0, s, kp, tb, p0, p1,
+                       nullptr, loff, lobs, lrloc);
+    return hipGetLastError();
+  }
   long long blocks = (p1 - p0 + 255) / 256;
   if (blocks < 1) return hipSuccess;
   if (blocks > 65536) blocks = 65536;
