@@ -720,6 +720,114 @@ __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int 
   }
 }
 
+// The K x K form (n >= K in the batched form, every point in the plain one)
+// of the BF16x3 Gram, one wavefront per point (nmem <= 64): C = B^T B with
+// B = diag(sqrt(w)) Y, w = rloc / rdiag, summed over the n obs in chunks of
+// 16.  Lane (h, c) holds members c and 32 + c of obs rows 16 k + 8 h .. + 7
+// (the 32 lanes of a half read a row's members contiguously): as the A
+// operand of its row tile (A[member][obs]) and the B operand of its column
+// tile (B[obs][member]) these are the same registers.  The row data (obs
+// index, sqrt weight, sqrt weight x innovation) of 64 rows at a time is
+// fetched one row per lane into a per-wavefront LDS table, so a chunk waits
+// on one memory latency (its rows' members) instead of three; b =
+// Y^T diag(w) dep sums in FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gram_wave_kk(int K, double diag, ObsSource<T> src, int na,
+                                                      double* __restrict__ G, double* __restrict__ bvec, int dual) {
+  struct RowMeta {
+    double sw, swd;
+  };
+  __shared__ RowMeta meta[4][64];
+  __shared__ int mio[4][64];
+  const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x * 4 + w;
+  if (b >= na) return;
+  const int n = src.begin(b);
+  if (dual && n < K) return;  // dual form: k_gram_wave
+  const bool two = K > 32;
+  const int m0 = c < K ? c : K - 1, m1 = 32 + c < K ? 32 + c : K - 1;
+  const bool ok0 = c < K, ok1 = 32 + c < K;
+  mf_v16 acc00 = mf_v16{}, acc01 = mf_v16{}, acc11 = mf_v16{};
+  double bp0 = 0.0, bp1 = 0.0;
+  auto split = [](double x, mf_bf8& hi, mf_bf8& lo, int e) {
+    const float v = (float)x;
+    const __bf16 vh = (__bf16)v;
+    hi[e] = vh;
+    lo[e] = (__bf16)(v - (float)vh);
+  };
+  for (int g0 = 0; g0 < n; g0 += 64) {
+    {
+      const int l = g0 + lane;
+      int io;
+      double wt, d;
+      src.wrow(l < n ? l : n - 1, io, wt, d);
+      const double sw = l < n ? sqrt(wt) : 0.0;
+      meta[w][lane] = RowMeta{sw, sw * d};
+      mio[w][lane] = io;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int gend = n - g0 < 64 ? n - g0 : 64;
+    for (int l0 = 0; l0 < gend; l0 += 16) {
+      mf_bf8 x0h, x0l, x1h, x1l;
+      double y0[8], y1[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const double* yr = src.yT + (long long)mio[w][l0 + 8 * h + e] * K;
+        y0[e] = yr[m0];
+        y1[e] = two ? yr[m1] : 0.0;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const RowMeta rm = meta[w][l0 + 8 * h + e];
+        const double v0 = ok0 ? rm.sw * y0[e] : 0.0;
+        bp0 = fma(v0, rm.swd, bp0);
+        split(v0, x0h, x0l, e);
+        if (two) {
+          const double v1 = ok1 ? rm.sw * y1[e] : 0.0;
+          bp1 = fma(v1, rm.swd, bp1);
+          split(v1, x1h, x1l, e);
+        }
+      }
+      acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x0h, acc00, 0, 0, 0);
+      acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x0l, acc00, 0, 0, 0);
+      acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0l, x0h, acc00, 0, 0, 0);
+      if (two) {
+        acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x1h, acc01, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x1l, acc01, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0l, x1h, acc01, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1h, x1h, acc11, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1h, x1l, acc11, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1l, x1h, acc11, 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the next 64 rows overwrite the table
+  }
+  // b for members c and 32 + c: the two row halves of the obs chunks
+  bp0 += __shfl_xor(bp0, 32);
+  bp1 += __shfl_xor(bp1, 32);
+  if (h == 0) {
+    if (ok0) bvec[(long long)b * K + c] = bp0;
+    if (two && ok1) bvec[(long long)b * K + 32 + c] = bp1;
+  }
+  double* Gp = G + (long long)b * K * K;
+  auto store = [&](const mf_v16& v, int tr, int tc) {
+    const int col = tc + c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = tr + (r >> 2) * 8 + h * 4 + (r & 3);
+      if (row < K && col < K && row <= col) Gp[row * K + col] = (double)v[r] + (row == col ? diag : 0.0);
+    }
+  };
+  store(acc00, 0, 0);
+  if (two) {
+    store(acc01, 0, 32);
+    store(acc11, 32, 32);
+  }
+}
+
 template <int PREC, typename T>
 static void launch_gram_mixed_p(int K, double diag, const ObsSource<T>& src, int na, double* G, double* bvec,
                                 int dual, hipStream_t s) {
@@ -740,10 +848,11 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
   if (kp.K > 128) return hipErrorInvalidValue;
   ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
-  if (prec == LETKF_PREC_BF16X3 && kp.K <= 64 && dual == 1 && !getenv("LETKF_GRAM_BLOCK")) {
-    // dual-form points one wavefront each, then the K x K ones
-    hipLaunchKernelGGL(k_gram_wave<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
-    dual = 2;
+  if (prec == LETKF_PREC_BF16X3 && kp.K <= 64 && dual != 2 && !getenv("LETKF_GRAM_BLOCK")) {
+    // one wavefront per point: the dual-form points, then the K x K ones
+    if (dual) hipLaunchKernelGGL(k_gram_wave<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
+    hipLaunchKernelGGL(k_gram_wave_kk<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, diag, src, na, G, bvec, dual);
+    return hipGetLastError();
   }
   if (prec == LETKF_PREC_FP32)
     launch_gram_mixed_p<LETKF_PREC_FP32, T>(kp.K, diag, src, na, G, bvec, dual, s);
