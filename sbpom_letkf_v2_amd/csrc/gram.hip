@@ -457,6 +457,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: tile tables in scalar registers
   const bool dm = dual && n < K;
   if (dual == 2 && dm) return;  // K x K slots only (the FP64 kernel takes the dual ones)
+  if (dual == 3 && dm && n <= 64) return;  // dual slots up to 64 rows: k_gram_wave
   const int N = dm ? n : K;  // output dimension
   const int S = dm ? K : n;  // summation length
   // this wavefront's tiles: row / column offsets and liveness (wave-uniform)
@@ -646,7 +647,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
 // of its column tile (A = B^T), so each value is split into BF16 hi + lo
 // once.  Operand values, their splits and the MFMA sequence are those of
 // k_gram_mfma<LETKF_PREC_BF16X3, 64> (upper-triangle tiles only); the K x K
-// points are left to it (dual == 2 there).
+// points (and, for nmem > 64, the dual ones of more than 64 rows) are left
+// to k_gram_wave_kk or k_gram_mfma.
 template <typename T>
 __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int na, double* __restrict__ G,
                                                    double* __restrict__ bvec) {
@@ -654,7 +656,7 @@ __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int 
   const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (b >= na) return;
   const int n = src.begin(b);
-  if (n >= K) return;  // K x K form: k_gram_mfma
+  if (n >= K || n > 64) return;  // K x K form, or more than two row tiles: k_gram_mfma
   const bool two = n > 32;  // tiles (0,0), (0,1), (1,1); else (0,0) only
   mf_v16 acc00 = mf_v16{}, acc01 = mf_v16{}, acc11 = mf_v16{};
   // x -> (hi, lo) in element e of two operand vectors
@@ -848,11 +850,17 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
   if (kp.K > 128) return hipErrorInvalidValue;
   ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
-  if (prec == LETKF_PREC_BF16X3 && kp.K <= 64 && dual != 2 && !getenv("LETKF_GRAM_BLOCK")) {
-    // one wavefront per point: the dual-form points, then the K x K ones
+  if (prec == LETKF_PREC_BF16X3 && dual != 2 && !getenv("LETKF_GRAM_BLOCK")) {
+    // one wavefront per point: the dual-form points of up to 64 rows, then
+    // (nmem <= 64) the K x K ones; for nmem > 64 the block kernel takes the
+    // rest (dual == 3: it skips the points done here)
     if (dual) hipLaunchKernelGGL(k_gram_wave<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
-    hipLaunchKernelGGL(k_gram_wave_kk<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, diag, src, na, G, bvec, dual);
-    return hipGetLastError();
+    if (kp.K <= 64) {
+      hipLaunchKernelGGL(k_gram_wave_kk<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, diag, src, na, G, bvec,
+                         dual);
+      return hipGetLastError();
+    }
+    if (dual) dual = 3;
   }
   if (prec == LETKF_PREC_FP32)
     launch_gram_mixed_p<LETKF_PREC_FP32, T>(kp.K, diag, src, na, G, bvec, dual, s);
