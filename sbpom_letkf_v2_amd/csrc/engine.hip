@@ -445,12 +445,15 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     const double ncol = 1.0 + p.nv3d + p.nv2d;
     const double log_bytes =
         rat ? 0.0 : 16.0 * eigr_rot_capacity(K) + (double)eigr_ix_pitch(K) + 8.0 * eigr_sweep_capacity(K);
-    const double per_slot = 16.0 * K * K + 8.0 * K * ncol + log_bytes + 64.0 * K + 8.0;
+    // point-major forecast + analysis copies: per point of the chunk, counted
+    // per slot (a chunk has at least as many points as active slots)
+    const double pm_bytes = 2.0 * sizeof(T) * K * (ncol - 1);
+    const double per_slot = 16.0 * K * K + 8.0 * K * ncol + log_bytes + 64.0 * K + 8.0 + pm_bytes;
     size_t mfree = 0, mtotal = 0;
     HIP_TRY(hipMemGetInfo(&mfree, &mtotal));
     const double held = (double)(h->G.bytes + h->V.bytes + h->yvec.bytes + h->lam.bytes + h->bvec.bytes +
                                  h->eD.bytes + h->eE.bytes + h->logc.bytes + h->logs.bytes + h->logi.bytes +
-                                 h->logn.bytes + h->logx.bytes + h->logr.bytes);
+                                 h->logn.bytes + h->logx.bytes + h->logr.bytes + h->xpm.bytes + h->apm.bytes);
     double budget = std::min(96e9, std::max(1e9, 0.45 * ((double)mfree + held)));
     if (p.workspace_bytes > 0)  // the caller's cap (room for several handles, or for the caller's own data)
       budget = std::min(std::max(1e9, (double)p.workspace_bytes), 0.9 * ((double)mfree + held));
@@ -545,6 +548,14 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
       HIP_TRY(h->logr.ensure(sizeof(int32_t) * (size_t)(maxA + 1)));
     }
   }
+  // point-major copies of each chunk's ensemble for the register update
+  // (test hook LETKF_NO_POINT_MAJOR: the update on the caller's arrays)
+  pl.pm_ld = 0;
+  if (reg_eig && !getenv("LETKF_NO_POINT_MAJOR")) {
+    pl.pm_ld = K * (p.nv3d + p.nv2d);
+    HIP_TRY(h->xpm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
+    HIP_TRY(h->apm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
+  }
   if (rat && !h->ratt_ok) {  // the coefficient table, once per handle (host copy built once per process)
     HIP_TRY(h->ratt.ensure(sizeof(double) * rat_table_size()));
     HIP_TRY(hipMemcpyAsync(h->ratt.ptr, rat_table_data(), sizeof(double) * rat_table_size(),
@@ -626,6 +637,12 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
                                  h->G.as<double>(), h->bvec.as<double>(), dual, s));
     }
     HIP_TRY(mark(e0 + 2));
+    PointMajor<T> pm{nullptr, nullptr, p0, pl.pm_ld};
+    if (na > 0 && reg_eig && pl.pm_ld > 0) {
+      pm.X = h->xpm.as<T>();
+      pm.A = h->apm.as<T>();
+      HIP_TRY(launch_to_point_major<T>(kp, p0, p1, f3, f2, h->xpm.as<T>(), pl.pm_ld, s));
+    }
     if (na > 0) {
       if (reg_eig) {
         for (int q = 0; q < kNQ; ++q) {
@@ -641,7 +658,7 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
                                         h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
                                         h->status.as<int>(), h->vpath[q],
                                         pl.rat ? h->ratt.as<double>() : nullptr,
-                                        p.precision == LETKF_PREC_BF16X3_TRI32, ktm, s));
+                                        p.precision == LETKF_PREC_BF16X3_TRI32, ktm, pm, s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -651,6 +668,7 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
       }
     }
     HIP_TRY(mark(e0 + 3));
+    if (pm.A) HIP_TRY(launch_from_point_major<T>(kp, p0, p1, pm.A, pl.pm_ld, h->nloc.as<int32_t>(), a3, a2, s));
     HIP_TRY(launch_transform<T>(kp, tb, p0, p1, h->nloc.as<int32_t>(), h->actidx.as<int32_t>(), a0,
                                 h->lam.as<double>(), h->V.as<double>(), h->bvec.as<double>(), f3, f2, a3, a2, dual,
                                 h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(),
@@ -1023,6 +1041,13 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
       HIP_TRY(launch_gram_any<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
                                  h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
                                  h->G.as<double>(), h->bvec.as<double>(), dual, s));
+      // the first chunk's point-major forecast, as the analysis uses it
+      PointMajor<T> pm{nullptr, nullptr, pl.bpts[0], pl.pm_ld};
+      if (pl.pm_ld > 0) {
+        pm.X = h->xpm.as<T>();
+        pm.A = h->apm.as<T>();
+        HIP_TRY(launch_to_point_major<T>(kp, pl.bpts[0], pl.bpts[1], f3, f2, h->xpm.as<T>(), pl.pm_ld, s));
+      }
       hipEvent_t e0, e1;
       HIP_TRY(hipEventCreate(&e0));
       HIP_TRY(hipEventCreate(&e1));
@@ -1043,7 +1068,7 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
                                           h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
                                           h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
                                           h->status.as<int>(), v, pl.rat ? h->ratt.as<double>() : nullptr,
-                                          p.precision == LETKF_PREC_BF16X3_TRI32, nullptr, s));
+                                          p.precision == LETKF_PREC_BF16X3_TRI32, nullptr, pm, s));
             HIP_TRY(hipEventRecord(e1, s));
             HIP_TRY(hipEventSynchronize(e1));
             float t = 0.f;

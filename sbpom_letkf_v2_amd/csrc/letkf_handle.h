@@ -59,6 +59,7 @@ struct AnalysisPlan {
   bool reg_eig = false;
   int dual = 0;
   bool rat = false;  // f(T) by rational functions (batched form, K <= 128)
+  int pm_ld = 0;     // register update through point-major chunk copies: their row pitch (0: none)
   std::vector<long long> bpts, bv;  // chunk point bounds; per-chunk (local obs, active slot) offsets + totals
   std::vector<int32_t> hist, boff;  // per (chunk, size bucket) slot counts and list offsets
   letkf_stats_t st{};               // counters known from the plan
@@ -94,6 +95,7 @@ struct letkf_handle {
   DevBuf adj_part;  // adjoint: per-point (dL/dalpha, dL/drho, dL/dsigma_obs, dL/dsigma_obsv), block sums
   DevBuf adj_pair, adj_sort;  // adjoint: per-pair terms; sort keys, values and scratch
   DevBuf bhist, bcur, blist;              // per-chunk size buckets of active slots
+  DevBuf xpm, apm;                        // point-major forecast / analysis of a chunk (register update)
   // staging for the host-pointer entry point
   DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
   letkf_stats_t stats{};
@@ -120,7 +122,7 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &ratt, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &ratt, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &xpm, &apm, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
     ratt_ok = false;
     for (hipEvent_t e : events) (void)hipEventDestroy(e);

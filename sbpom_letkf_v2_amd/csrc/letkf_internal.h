@@ -181,6 +181,24 @@ struct EigTimer {
   unsigned long long* rot;  // device counter: logged rotations of every slot (may be null)
 };
 
+// Point-major copy of a chunk's ensemble for the register update: member m
+// of variable v at point p in X[(p - p0) ld + v K + m], ld = (nv3d + nv2d) K
+// (the 2-D variables at the surface points only).  X null: the update reads
+// the caller's arrays; A null: it writes them.
+template <typename T>
+struct PointMajor {
+  const T* X;
+  T* A;
+  long long p0;
+  int ld;
+};
+template <typename T>
+hipError_t launch_to_point_major(const KParams& kp, long long p0, long long p1, const T* X3, const T* X2,
+                                 T* Xp, int ld, hipStream_t s);
+template <typename T>
+hipError_t launch_from_point_major(const KParams& kp, long long p0, long long p1, const T* Ap, int ld,
+                                   const int32_t* nloc, T* A3, T* A2, hipStream_t s);
+
 long long eigr_rot_capacity(int K);
 long long eigr_sweep_capacity(int K);
 int eigr_vpath(int vpath, int nb, int K, int ncol);
@@ -198,7 +216,7 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const int32_t* lobs, const double* lrloc,
                               const double* yT, const double* bvec,
                               int* status, int vpath, const double* rat, bool tri32, EigTimer* tm,
-                              hipStream_t s);
+                              const PointMajor<T>& pm, hipStream_t s);
 // The rational path's coefficient table (ratfun.inc), rat_table_size()
 // doubles, built once per process; rat == nullptr above selects E2 + V2.
 // Layout: [kind][class][n, d, a[kRatTablePoles], s[kRatTablePoles]].

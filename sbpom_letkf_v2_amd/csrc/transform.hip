@@ -461,6 +461,100 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
   }
 }
 
+// ---------------------------------------------------------------- point-major copies
+// The register update reads every member of every variable of one point and
+// writes them back; in the caller's layout (point index fastest) those are
+// K (nv3d + nv2d) separate cache lines per point, in a point-major copy a few
+// contiguous rows.  The copies run per chunk as 64 x 64 (points x members)
+// tiles through LDS, both sides coalesced: forecast in before the update,
+// analysis out after it (only points with local obs: the others are written
+// by k_noobs, and the fallback slots by the tile transform after this).
+template <typename T>
+__global__ void __launch_bounds__(256) k_to_point_major(KParams kp, long long p0, long long p1, const T* __restrict__ X3,
+                                                        const T* __restrict__ X2, T* __restrict__ Xp, int ld) {
+  __shared__ T tile[64][65];
+  const int v = blockIdx.z, K = kp.K, nv3 = kp.nv3d;
+  const long long pb = p0 + (long long)blockIdx.x * 64;
+  const int mb = blockIdx.y * 64;
+  const long long surf0 = (long long)kp.nij1 * (kp.nlev - 1);
+  if (v >= nv3 && pb + 63 < surf0) return;  // 2-D variables: surface points only
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int m = mb + r;
+    const long long p = pb + tx;
+    T x = T(0);
+    if (m < K && p < p1) {
+      if (v < nv3)
+        x = X3[p + kp.P * (m + (long long)K * v)];
+      else if (p >= surf0)
+        x = X2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))];
+    }
+    tile[r][tx] = x;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const long long p = pb + r;
+    const int m = mb + tx;
+    if (p < p1 && m < K && (v < nv3 || p >= surf0)) Xp[(p - p0) * ld + (long long)K * v + m] = tile[tx][r];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_from_point_major(KParams kp, long long p0, long long p1,
+                                                          const T* __restrict__ Ap, int ld,
+                                                          const int32_t* __restrict__ nloc, T* __restrict__ A3,
+                                                          T* __restrict__ A2) {
+  __shared__ T tile[64][65];
+  const int v = blockIdx.z, K = kp.K, nv3 = kp.nv3d;
+  const long long pb = p0 + (long long)blockIdx.x * 64;
+  const int mb = blockIdx.y * 64;
+  const long long surf0 = (long long)kp.nij1 * (kp.nlev - 1);
+  if (v >= nv3 && pb + 63 < surf0) return;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const long long p = pb + r;
+    const int m = mb + tx;
+    tile[tx][r] = p < p1 && m < K && (v < nv3 || p >= surf0) ? Ap[(p - p0) * ld + (long long)K * v + m] : T(0);
+  }
+  __syncthreads();
+  const long long p = pb + tx;
+  const bool act = p < p1 && nloc[p] > 0;
+  for (int r = ty; r < 64; r += 4) {
+    const int m = mb + r;
+    if (!act || m >= K) continue;
+    if (v < nv3)
+      A3[p + kp.P * (m + (long long)K * v)] = tile[r][tx];
+    else if (p >= surf0)
+      A2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))] = tile[r][tx];
+  }
+}
+
+template <typename T>
+hipError_t launch_to_point_major(const KParams& kp, long long p0, long long p1, const T* X3, const T* X2, T* Xp,
+                                 int ld, hipStream_t s) {
+  if (p1 <= p0) return hipSuccess;
+  const dim3 grid((unsigned)((p1 - p0 + 63) / 64), (unsigned)((kp.K + 63) / 64), (unsigned)(kp.nv3d + kp.nv2d));
+  hipLaunchKernelGGL(k_to_point_major<T>, grid, dim3(256), 0, s, kp, p0, p1, X3, X2, Xp, ld);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_from_point_major(const KParams& kp, long long p0, long long p1, const T* Ap, int ld,
+                                   const int32_t* nloc, T* A3, T* A2, hipStream_t s) {
+  if (p1 <= p0) return hipSuccess;
+  const dim3 grid((unsigned)((p1 - p0 + 63) / 64), (unsigned)((kp.K + 63) / 64), (unsigned)(kp.nv3d + kp.nv2d));
+  hipLaunchKernelGGL(k_from_point_major<T>, grid, dim3(256), 0, s, kp, p0, p1, Ap, ld, nloc, A3, A2);
+  return hipGetLastError();
+}
+template hipError_t launch_to_point_major<float>(const KParams&, long long, long long, const float*, const float*,
+                                                 float*, int, hipStream_t);
+template hipError_t launch_to_point_major<double>(const KParams&, long long, long long, const double*,
+                                                  const double*, double*, int, hipStream_t);
+template hipError_t launch_from_point_major<float>(const KParams&, long long, long long, const float*, int,
+                                                   const int32_t*, float*, float*, hipStream_t);
+template hipError_t launch_from_point_major<double>(const KParams&, long long, long long, const double*, int,
+                                                    const int32_t*, double*, double*, hipStream_t);
+
 template <typename T>
 hipError_t launch_transform(const KParams& kp, const Tables<T>& tb, long long p0, long long p1,
                             const int32_t* nloc, const int32_t* actidx, int a0, const double* lam,
