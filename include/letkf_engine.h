@@ -531,8 +531,10 @@ int letkf_get_nobsl(letkf_handle_t* h, int32_t* nobsl, void* stream);
 /* One entry of the coefficient table of the batched form's matrix
  * functions (LETKF_FORM_BATCHED): Zolotarev's rational approximation
  * x^-1/2 ~ d + sum_l a[l] / (x + s[l]) on [1, 2^(cls+1)], kind 0 (the
- * nmem x nmem form, max relative error < 1e-15) or 1 (the nobsl x nobsl
- * form, < 1e-16).  Writes d and the poles' a, s (up to 40 each) and
+ * nmem x nmem form, max relative error < 1e-15), 1 (the nobsl x nobsl
+ * form, < 1e-16), 2 and 3 (the same forms after the FP32 reduction of
+ * precision LETKF_PREC_BF16X3_TRI32, < 1e-10 and < 1e-11).  Writes d and
+ * the poles' a, s (up to 40 each) and
  * returns their count (0: no entry; the class's matrices go to the
  * fallback), or a negative status.  No counterpart in the reference (it
  * uses the eigen-decomposition, common_letkf.f90:132-208); exposed for

@@ -137,8 +137,8 @@ extern "C" int letkf_get_nobsl(letkf_handle_t* h, int32_t* nobsl, void* stream) 
 
 extern "C" int letkf_rational_table_entry(int32_t kind, int32_t cls, double* d, double* a, double* s) {
   if (!d || !a || !s) return set_err(LETKF_ERR_ARG, "NULL argument");
-  if (kind < 0 || kind > 1 || cls < 0 || cls >= kRatTableClasses)
-    return set_err(LETKF_ERR_ARG, "kind must be 0 or 1 and cls in [0,%d)", kRatTableClasses);
+  if (kind < 0 || kind >= kRatTableKinds || cls < 0 || cls >= kRatTableClasses)
+    return set_err(LETKF_ERR_ARG, "kind must be in [0,%d) and cls in [0,%d)", kRatTableKinds, kRatTableClasses);
   const double* e = rat_table_data() + ((size_t)kind * kRatTableClasses + cls) * (2 + 2 * kRatTablePoles);
   const int n = (int)e[0];
   *d = e[1];

@@ -219,8 +219,9 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const PointMajor<T>& pm, hipStream_t s);
 // The rational path's coefficient table (ratfun.inc), rat_table_size()
 // doubles, built once per process; rat == nullptr above selects E2 + V2.
-// Layout: [kind][class][n, d, a[kRatTablePoles], s[kRatTablePoles]].
-constexpr int kRatTableClasses = 24, kRatTablePoles = 40;
+// Layout: [kind][class][n, d, a[kRatTablePoles], s[kRatTablePoles]], kinds
+// primal, dual, and the two after an FP32 reduction (looser targets).
+constexpr int kRatTableClasses = 24, kRatTablePoles = 40, kRatTableKinds = 4;
 const double* rat_table_data();
 size_t rat_table_size();
 // Zolotarev's approximation of x^-1/2 on [1, kappa] of order n: d, a[n],
