@@ -328,15 +328,16 @@ template <typename T>
 static hipError_t launch_gram_any(int prec, const KParams& kp, const Tables<T>& tb, const double* yT,
                                   const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
                                   const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
-                                  hipStream_t s) {
+                                  const void* ysp, hipStream_t s) {
   if (prec == LETKF_PREC_BF16X3_TRI32) prec = LETKF_PREC_BF16X3;  // the Gram of the mode
   if (prec == LETKF_PREC_FP64 || kp.K > 128)
     return launch_gram_fp64<T>(kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec, dual, s);
   if (prec != LETKF_PREC_FP32_DUAL64 || !dual)
     return launch_gram_mixed<T>(prec == LETKF_PREC_FP32_DUAL64 ? LETKF_PREC_FP32 : prec, kp, tb, yT, act_p, loff,
-                                lbase, lobs, lrloc, a0, na, G, bvec, dual, s);
+                                lbase, lobs, lrloc, a0, na, G, bvec, dual,
+                                prec == LETKF_PREC_BF16X3 ? ysp : nullptr, s);
   hipError_t e = launch_gram_mixed<T>(LETKF_PREC_FP32, kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec,
-                                      2, s);
+                                      2, nullptr, s);
   if (e != hipSuccess) return e;
   return launch_gram_fp64<T>(kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec, 2, s);
 }
@@ -556,6 +557,11 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     HIP_TRY(h->xpm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
     HIP_TRY(h->apm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
   }
+  // the dual-form BF16x3 Gram's operands split once per analysis (test hook
+  // LETKF_GRAM_NOSPLIT: split per gather from the FP64 rows)
+  pl.ysplit = dual && (p.precision == LETKF_PREC_BF16X3 || p.precision == LETKF_PREC_BF16X3_TRI32) && K <= 128 &&
+              !getenv("LETKF_GRAM_BLOCK") && !getenv("LETKF_GRAM_NOSPLIT");
+  if (pl.ysplit) HIP_TRY(h->ysp.ensure((size_t)32 * split_obs_groups(K) * (size_t)o->nobs));
   if (rat && !h->ratt_ok) {  // the coefficient table, once per handle (host copy built once per process)
     HIP_TRY(h->ratt.ensure(sizeof(double) * rat_table_size()));
     HIP_TRY(hipMemcpyAsync(h->ratt.ptr, rat_table_data(), sizeof(double) * rat_table_size(),
@@ -617,8 +623,10 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
   const std::vector<long long>& bpts = pl.bpts;
   const std::vector<long long>& bv = pl.bv;
   auto mark = [&](size_t q) -> hipError_t { return timing ? hipEventRecord(h->events[q], s) : hipSuccess; };
-  // obs-major FP64 copy of obshdxf
+  // obs-major FP64 copy of obshdxf (and its BF16 split for the dual Gram)
   HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
+  const void* ysp = pl.ysplit ? h->ysp.ptr : nullptr;
+  if (ysp) HIP_TRY(launch_split_obs(o->nobs, K, h->yT.as<double>(), h->ysp.ptr, s));
   for (long long c = 0; c < nch; ++c) {
     const size_t e0 = 2 + 5 * (size_t)c;
     const long long p0 = bpts[c], p1 = bpts[c + 1];
@@ -634,7 +642,7 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
     if (na > 0) {
       HIP_TRY(launch_gram_any<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
                                  h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
-                                 h->G.as<double>(), h->bvec.as<double>(), dual, s));
+                                 h->G.as<double>(), h->bvec.as<double>(), dual, ysp, s));
     }
     HIP_TRY(mark(e0 + 2));
     PointMajor<T> pm{nullptr, nullptr, p0, pl.pm_ld};
@@ -1036,11 +1044,13 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
       hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, h->status.as<int>());
       HIP_TRY(hipGetLastError());
       HIP_TRY(launch_transpose_obs<T>(o->nobs, K, (const T*)o->hdxf, h->yT.as<double>(), s));
+      const void* ysp = pl.ysplit ? h->ysp.ptr : nullptr;
+      if (ysp) HIP_TRY(launch_split_obs(o->nobs, K, h->yT.as<double>(), h->ysp.ptr, s));
       HIP_TRY(launch_search_fill<T>(kp, tb, pl.bpts[0], pl.bpts[1], h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
                                     h->lrloc.as<double>(), s));
       HIP_TRY(launch_gram_any<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
                                  h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
-                                 h->G.as<double>(), h->bvec.as<double>(), dual, s));
+                                 h->G.as<double>(), h->bvec.as<double>(), dual, ysp, s));
       // the first chunk's point-major forecast, as the analysis uses it
       PointMajor<T> pm{nullptr, nullptr, pl.bpts[0], pl.pm_ld};
       if (pl.pm_ld > 0) {

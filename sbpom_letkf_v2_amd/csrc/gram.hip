@@ -12,6 +12,8 @@
 
 namespace letkf {
 
+typedef float mf_v16 __attribute__((ext_vector_type(16)));
+typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
 
 // Where the rows of one local problem come from.
 //  ObsSource: the da_main path -- row l of point p is obs lobs[o0+l] of the
@@ -29,6 +31,7 @@ struct ObsSource {
   const double* lrloc;
   int a0;
   long long o0;
+  const mf_bf8* ysp = nullptr;  // the rows as BF16 hi / lo groups (launch_split_obs), or none
   __device__ int begin(int b) {
     const int p = act_p[a0 + b];
     o0 = loff[p] - lbase;
@@ -415,8 +418,6 @@ template hipError_t launch_gram_fp64<double>(const KParams&, const Tables<double
 //   32x32x2 f32 : a = A[i][k], i = lane%32, k = lane/32;  b = B[k][j], j = lane%32
 //   32x32x16 bf16: a = A[i][8*(lane/32)+q], b = B[8*(lane/32)+q][j], q = 0..7
 //   C (16 floats): C[(r/4)*8 + (lane/32)*4 + r%4][lane%32]
-typedef float mf_v16 __attribute__((ext_vector_type(16)));
-typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMfPanel = 32;  // obs (primal) or members (dual) per LDS panel
 
@@ -649,11 +650,18 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
 // k_gram_mfma<LETKF_PREC_BF16X3, 64> (upper-triangle tiles only); the K x K
 // points (and, for nmem > 64, the dual ones of more than 64 rows) are left
 // to k_gram_wave_kk or k_gram_mfma.
-template <typename T>
+// SPLIT: the operands are the unweighted rows, pre-split once per analysis
+// (launch_split_obs: one 32-byte load gives a lane its hi and lo vectors,
+// half the bytes of the FP64 row and no conversion), and the weights come
+// in at the store, M(i, j) = sw_i sw_j (Y Y^T)(i, j) -- the same matrix,
+// the BF16 splits taken of y instead of sw y.
+template <typename T, bool SPLIT>
 __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int na, double* __restrict__ G,
                                                    double* __restrict__ bvec) {
+  __shared__ double swl[SPLIT ? 4 : 1][64];  // SPLIT: the rows' sqrt weights
   const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5;
-  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x * 4 + w;
   if (b >= na) return;
   const int n = src.begin(b);
   if (n >= K || n > 64) return;  // K x K form, or more than two row tiles: k_gram_mfma
@@ -679,23 +687,42 @@ __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int 
   }
   const double* y0 = src.yT + (long long)io0 * K;
   const double* y1 = src.yT + (long long)io1 * K;
+  const int ng = split_obs_groups(K);
+  const mf_bf8* s0 = SPLIT ? src.ysp + ((long long)io0 * ng + h) * 2 : nullptr;
+  const mf_bf8* s1 = SPLIT ? src.ysp + ((long long)io1 * ng + h) * 2 : nullptr;
+  if (SPLIT) {
+    swl[w][h * 32 + c] = h == 0 ? sw0 : sw1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
   for (int k0 = 0; k0 < K; k0 += 16) {
     mf_bf8 x0h, x0l, x1h, x1l;
+    if (SPLIT) {
+      x0h = s0[k0 / 4];
+      x0l = s0[k0 / 4 + 1];
+      if (two) {
+        x1h = s1[k0 / 4];
+        x1l = s1[k0 / 4 + 1];
+      }
+    } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int m = k0 + 8 * h + e;
-      const double v0 = m < K ? sw0 * y0[m < K ? m : K - 1] : 0.0;
-      split(v0, x0h, x0l, e);
+      for (int e = 0; e < 8; ++e) {
+        const int m = k0 + 8 * h + e;
+        const double v0 = m < K ? sw0 * y0[m < K ? m : K - 1] : 0.0;
+        split(v0, x0h, x0l, e);
+      }
     }
     acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x0h, acc00, 0, 0, 0);
     acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x0l, acc00, 0, 0, 0);
     acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0l, x0h, acc00, 0, 0, 0);
     if (two) {
+      if (!SPLIT) {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int m = k0 + 8 * h + e;
-        const double v1 = m < K ? sw1 * y1[m < K ? m : K - 1] : 0.0;
-        split(v1, x1h, x1l, e);
+        for (int e = 0; e < 8; ++e) {
+          const int m = k0 + 8 * h + e;
+          const double v1 = m < K ? sw1 * y1[m < K ? m : K - 1] : 0.0;
+          split(v1, x1h, x1l, e);
+        }
       }
       acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x1h, acc01, 0, 0, 0);
       acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0h, x1l, acc01, 0, 0, 0);
@@ -709,10 +736,12 @@ __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int 
   double* Gp = G + (long long)b * K * K;
   auto store = [&](const mf_v16& v, int tr, int tc) {
     const int col = tc + c;
+    const double swc = SPLIT ? swl[w][col < 64 ? col : 63] : 1.0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = tr + (r >> 2) * 8 + h * 4 + (r & 3);
-      if (row < n && col < n && row <= col) Gp[row * n + col] = (double)v[r];
+      if (row < n && col < n && row <= col)
+        Gp[row * n + col] = SPLIT ? (double)v[r] * (swl[w][row] * swc) : (double)v[r];
     }
   };
   store(acc00, 0, 0);
@@ -720,6 +749,34 @@ __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int 
     store(acc01, 0, 32);
     store(acc11, 32, 32);
   }
+}
+
+// yT rows -> BF16 hi / lo groups of 8 members, one thread a group
+__global__ void __launch_bounds__(256) k_split_obs(int nobs, int K, int ng, const double* __restrict__ yT,
+                                                   mf_bf8* __restrict__ ysp) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)nobs * ng) return;
+  const long long io = t / ng;
+  const int g = (int)(t - io * ng);
+  mf_bf8 hi, lo;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int m = 8 * g + e;
+    const float v = m < K ? (float)yT[io * K + m] : 0.f;
+    const __bf16 vh = (__bf16)v;
+    hi[e] = vh;
+    lo[e] = (__bf16)(v - (float)vh);
+  }
+  ysp[2 * t] = hi;
+  ysp[2 * t + 1] = lo;
+}
+
+hipError_t launch_split_obs(int nobs, int K, const double* yT, void* ysp, hipStream_t s) {
+  const long long nt = (long long)nobs * split_obs_groups(K);
+  if (nt <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_split_obs, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, nobs, K,
+                     split_obs_groups(K), yT, static_cast<mf_bf8*>(ysp));
+  return hipGetLastError();
 }
 
 // The K x K form (n >= K in the batched form, every point in the plain one)
@@ -845,16 +902,19 @@ template <typename T>
 hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, const double* yT,
                              const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
                              const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
-                             hipStream_t s) {
+                             const void* ysp, hipStream_t s) {
   if (na <= 0) return hipSuccess;
   if (kp.K > 128) return hipErrorInvalidValue;
-  ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
+  ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0, static_cast<const mf_bf8*>(ysp)};
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
   if (prec == LETKF_PREC_BF16X3 && dual != 2 && !getenv("LETKF_GRAM_BLOCK")) {
     // one wavefront per point: the dual-form points of up to 64 rows, then
     // (nmem <= 64) the K x K ones; for nmem > 64 the block kernel takes the
     // rest (dual == 3: it skips the points done here)
-    if (dual) hipLaunchKernelGGL(k_gram_wave<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
+    if (dual && ysp)
+      hipLaunchKernelGGL((k_gram_wave<T, true>), dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
+    else if (dual)
+      hipLaunchKernelGGL((k_gram_wave<T, false>), dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
     if (kp.K <= 64) {
       hipLaunchKernelGGL(k_gram_wave_kk<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, diag, src, na, G, bvec,
                          dual);
@@ -874,9 +934,11 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
 }
 template hipError_t launch_gram_mixed<float>(int, const KParams&, const Tables<float>&, const double*,
                                              const int32_t*, const int64_t*, long long, const int32_t*,
-                                             const double*, int, int, double*, double*, int, hipStream_t);
+                                             const double*, int, int, double*, double*, int, const void*,
+                                             hipStream_t);
 template hipError_t launch_gram_mixed<double>(int, const KParams&, const Tables<double>&, const double*,
                                               const int32_t*, const int64_t*, long long, const int32_t*,
-                                              const double*, int, int, double*, double*, int, hipStream_t);
+                                              const double*, int, int, double*, double*, int, const void*,
+                                              hipStream_t);
 
 }  // namespace letkf

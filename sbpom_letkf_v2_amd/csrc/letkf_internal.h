@@ -84,13 +84,21 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
 // or from BF16 hi/lo splits (LETKF_PREC_BF16X3), FP32 accumulation (per
 // panel into FP32 totals for LETKF_PREC_FP32_PANEL), b in
 // FP64; K <= 128.
+// ysp (optional, BF16X3 only): the obs rows pre-split by launch_split_obs,
+// which the dual-form points' Gram then reads instead of yT.
 template <typename T>
 hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb,
                              const double* yT, const int32_t* act_p,
                              const int64_t* loff, long long lbase,
                              const int32_t* lobs, const double* lrloc, int a0,
                              int na, double* G, double* bvec, int dual,
-                             hipStream_t s);
+                             const void* ysp, hipStream_t s);
+
+// Obs rows of yT (nobs x K) as BF16 hi / lo pairs, 8 members a group
+// (zero past K), split_obs_groups(K) groups a row: 32 bytes a group, hi
+// then lo.
+__host__ __device__ inline int split_obs_groups(int K) { return 2 * ((K + 15) / 16); }
+hipError_t launch_split_obs(int nobs, int K, const double* yT, void* ysp, hipStream_t s);
 
 // Same products for letkf_core_batched: problem b owns rows offs[b]..offs[b+1)
 // of hdxf (K values per row); nrow[b] receives the row count.

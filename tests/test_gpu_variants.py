@@ -1,12 +1,15 @@
 """The kernel variants the defaults leave out, against the FP64 oracle.
 
-The column search, the one-wavefront Gram, the MFMA Householder reduction
-and the point-major copies are the defaults; the kernels they replaced stay
+The column search, the one-wavefront Gram (on pre-split rows in the dual
+form), the MFMA Householder reduction and the point-major copies are the
+defaults; the kernels they replaced stay
 in the library (the per-point search is also the path for nlev > 128) and
 are reached through switches read at plan / launch time:
 
   LETKF_SEARCH_POINT    local-obs count and fill one thread per point;
   LETKF_GRAM_BLOCK      the BF16x3 Gram on the four-wavefront block kernel;
+  LETKF_GRAM_NOSPLIT    the dual-form BF16x3 Gram splitting the FP64 rows per
+                        gather instead of reading the pre-split table;
   LETKF_E1_VALU         the FP32 reduction of the 32 / 64 buckets on the
                         vector ALUs;
   LETKF_NO_POINT_MAJOR  the register update on the caller's arrays.
@@ -54,7 +57,7 @@ def _mean_err(a3, a3o):
     return max(float(np.abs(m[..., v] - mo[..., v]).max() / np.abs(mo[..., v]).max()) for v in range(a3.shape[3]))
 
 
-@pytest.mark.parametrize("hook", ["LETKF_SEARCH_POINT", "LETKF_GRAM_BLOCK", "LETKF_E1_VALU",
+@pytest.mark.parametrize("hook", ["LETKF_SEARCH_POINT", "LETKF_GRAM_BLOCK", "LETKF_GRAM_NOSPLIT", "LETKF_E1_VALU",
                                   "LETKF_NO_POINT_MAJOR", None])
 @pytest.mark.parametrize("K", [64, 100])
 def test_variant_matches_oracle(monkeypatch, hook, K):
