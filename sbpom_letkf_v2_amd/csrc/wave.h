@@ -8,11 +8,21 @@ namespace letkf {
 // Wavefront sum of a double with DPP moves (no LDS traffic): butterfly
 // within quads, half-row and row mirrors, then the row broadcasts 15 and 31
 // fold the four rows into lane 63, which readlane hands to every lane.
+// With every row enabled (the permutations within a row: every lane reads
+// a valid lane) the move's "old" operand is never used and is left
+// undefined, so no zero is materialised first; the row broadcasts (rows
+// masked off) keep 0 in the lanes they do not write.
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ double dpp_d(double v) {
   const long long x = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, ROWMASK, 0xF, false);
+  int lo, hi;
+  if constexpr (ROWMASK == 0xF) {
+    lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xF, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, ROWMASK, 0xF, false);
+  }
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
@@ -53,7 +63,10 @@ __device__ __forceinline__ void wave_sum2(double& a, double& b) {
 // The same sums for floats (one DPP move per step).
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
+  if constexpr (ROWMASK == 0xF)
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+  else
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
