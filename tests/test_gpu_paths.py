@@ -257,3 +257,29 @@ def test_single_local_observation_points(form):
     e2 = per_var_err(a2, a2o, axis_var=2)
     _say(f"  err3={e3} err2={e2}")
     assert max(e3) <= 1e-10 and max(e2) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3t"])
+def test_degenerate_obs_ensemble(precision):
+    """Obs whose ensemble perturbations are zero or 1e-20 of the others:
+    Gram entries at and below the FP32 denormal range, columns the FP32
+    Householder steps must take as zero (a reflector skipped), against the
+    FP64 oracle."""
+    _say(f"case degenerate obs ensemble {precision}")
+    pr = synth.make_problem(nlon=24, nlat=20, nlev=5, nmem=40, nobs=600, seed=131, alpha_rtpp=0.5,
+                            real_kind=8, sigma_h_cells=1.5)
+    pr.obshdxf[:, 0::3] *= 1e-20   # (nmem, nobs): the Fortran (nobs, nmem) array
+    pr.obshdxf[:, 1::3] = 0.0
+    a3o, a2o, nl = _oracle(pr)
+    a3, a2, st = _engine(pr, precision, "batched")
+    assert st["local_obs"] == int(nl.sum())
+    e3 = per_var_err(a3, a3o, axis_var=3)
+    e2 = per_var_err(a2, a2o, axis_var=2)
+    _say(f"  err3={max(e3):.2e} err2={max(e2):.2e} buckets={st['bucket_slots']}")
+    if precision == "fp64":
+        assert max(e3) <= 1e-10 and max(e2) <= 1e-10
+    else:
+        assert max(e3) <= 1e-4 and max(e2) <= 1e-4
+        m, mo = a3.mean(axis=2), a3o.mean(axis=2)
+        em = max(float(np.abs(m[..., v] - mo[..., v]).max() / np.abs(mo[..., v]).max()) for v in range(a3.shape[3]))
+        assert em <= 1e-5
