@@ -479,20 +479,26 @@ __global__ void __launch_bounds__(256) k_to_point_major(KParams kp, long long p0
   const long long surf0 = (long long)kp.nij1 * (kp.nlev - 1);
   if (v >= nv3 && pb + 63 < surf0) return;  // 2-D variables: surface points only
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int r = ty; r < 64; r += 4) {
-    const int m = mb + r;
+  // the thread's 16 loads issued together, then the LDS stores
+  T x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int m = mb + ty + 4 * k;
     const long long p = pb + tx;
-    T x = T(0);
+    x[k] = T(0);
     if (m < K && p < p1) {
       if (v < nv3)
-        x = X3[p + kp.P * (m + (long long)K * v)];
+        x[k] = X3[p + kp.P * (m + (long long)K * v)];
       else if (p >= surf0)
-        x = X2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))];
+        x[k] = X2[(p - surf0) + (long long)kp.nij1 * (m + (long long)K * (v - nv3))];
     }
-    tile[r][tx] = x;
   }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) tile[ty + 4 * k][tx] = x[k];
   __syncthreads();
-  for (int r = ty; r < 64; r += 4) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int r = ty + 4 * k;
     const long long p = pb + r;
     const int m = mb + tx;
     if (p < p1 && m < K && (v < nv3 || p >= surf0)) Xp[(p - p0) * ld + (long long)K * v + m] = tile[tx][r];
@@ -511,15 +517,22 @@ __global__ void __launch_bounds__(256) k_from_point_major(KParams kp, long long 
   const long long surf0 = (long long)kp.nij1 * (kp.nlev - 1);
   if (v >= nv3 && pb + 63 < surf0) return;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int r = ty; r < 64; r += 4) {
-    const long long p = pb + r;
+  // the thread's 16 loads issued together, then the LDS stores
+  T x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const long long p = pb + ty + 4 * k;
     const int m = mb + tx;
-    tile[tx][r] = p < p1 && m < K && (v < nv3 || p >= surf0) ? Ap[(p - p0) * ld + (long long)K * v + m] : T(0);
+    x[k] = p < p1 && m < K && (v < nv3 || p >= surf0) ? Ap[(p - p0) * ld + (long long)K * v + m] : T(0);
   }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) tile[tx][ty + 4 * k] = x[k];
   __syncthreads();
   const long long p = pb + tx;
   const bool act = p < p1 && nloc[p] > 0;
-  for (int r = ty; r < 64; r += 4) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int r = ty + 4 * k;
     const int m = mb + r;
     if (!act || m >= K) continue;
     if (v < nv3)
