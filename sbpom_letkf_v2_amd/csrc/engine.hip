@@ -557,6 +557,9 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     HIP_TRY(h->xpm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
     HIP_TRY(h->apm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
   }
+  // rloc from the FP32 exponential in precision bf16x3t (test hook
+  // LETKF_EXACT_RLOC: the FP64 one)
+  pl.fastloc = p.precision == LETKF_PREC_BF16X3_TRI32 && !getenv("LETKF_EXACT_RLOC");
   // the dual-form BF16x3 Gram's operands split once per analysis (test hook
   // LETKF_GRAM_NOSPLIT: split per gather from the FP64 rows)
   pl.ysplit = dual && (p.precision == LETKF_PREC_BF16X3 || p.precision == LETKF_PREC_BF16X3_TRI32) && K <= 128 &&
@@ -636,7 +639,7 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
     HIP_TRY(mark(e0));
     if (na > 0) {
       HIP_TRY(launch_search_fill<T>(kp, tb, p0, p1, h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
-                                    h->lrloc.as<double>(), s));
+                                    h->lrloc.as<double>(), s, pl.fastloc));
     }
     HIP_TRY(mark(e0 + 1));
     if (na > 0) {
@@ -1047,7 +1050,7 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
       const void* ysp = pl.ysplit ? h->ysp.ptr : nullptr;
       if (ysp) HIP_TRY(launch_split_obs(o->nobs, K, h->yT.as<double>(), h->ysp.ptr, s));
       HIP_TRY(launch_search_fill<T>(kp, tb, pl.bpts[0], pl.bpts[1], h->loff.as<int64_t>(), h->lobs.as<int32_t>(),
-                                    h->lrloc.as<double>(), s));
+                                    h->lrloc.as<double>(), s, pl.fastloc));
       HIP_TRY(launch_gram_any<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
                                  h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
                                  h->G.as<double>(), h->bvec.as<double>(), dual, ysp, s));

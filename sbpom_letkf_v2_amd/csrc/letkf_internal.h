@@ -54,10 +54,12 @@ hipError_t launch_obs_trig(int nobs, double pi, const T* olat, double2* osc, hip
 template <typename T>
 hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb,
                                int32_t* nloc, hipStream_t s);
+// fast_loc: rloc from the FP32 exponential (v_exp_f32, ~1e-7 relative) in
+// place of the FP64 one -- the mixed-precision analysis (column kernel only)
 template <typename T>
 hipError_t launch_search_fill(const KParams& kp, const Tables<T>& tb,
                               long long p0, long long p1, const int64_t* loff,
-                              int32_t* lobs, double* lrloc, hipStream_t s);
+                              int32_t* lobs, double* lrloc, hipStream_t s, bool fast_loc = false);
 
 hipError_t launch_scan_points(long long P, const int32_t* nloc, int64_t* loff,
                               int32_t* actidx, int32_t* act_p, void* tmp,

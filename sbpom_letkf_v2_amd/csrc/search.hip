@@ -153,7 +153,7 @@ __device__ __forceinline__ long long readlane_ll(long long x, int l) {
   return ((long long)hi << 32) | (unsigned)lo;
 }
 
-template <typename T, bool FILL>
+template <typename T, bool FILL, bool FASTLOC = false>
 __global__ void __launch_bounds__(256) k_search_col(KParams kp, Tables<T> tb, long long p0, long long p1,
                                                     int32_t* __restrict__ nloc, const int64_t* __restrict__ loff,
                                                     int32_t* __restrict__ lobs, double* __restrict__ lrloc) {
@@ -273,7 +273,10 @@ __global__ void __launch_bounds__(256) k_search_col(KParams kp, Tables<T> tb, lo
             T q = (dist / so) * (dist / so);
             if (vloc) q = q + (dlev / sv) * (dlev / sv);
             lobs[pos] = io;
-            lrloc[pos] = (double)(T)exp(-0.5 * (double)q);
+            if (FASTLOC)
+              lrloc[pos] = (double)__expf(-0.5f * (float)q);
+            else
+              lrloc[pos] = (double)(T)exp(-0.5 * (double)q);
           }
         }
         const int nacc = __popcll(m);
@@ -330,11 +333,15 @@ hipError_t launch_search_count(const KParams& kp, const Tables<T>& tb, int32_t* 
 
 template <typename T>
 hipError_t launch_search_fill(const KParams& kp, const Tables<T>& tb, long long p0, long long p1,
-                              const int64_t* loff, int32_t* lobs, double* lrloc, hipStream_t s) {
+                              const int64_t* loff, int32_t* lobs, double* lrloc, hipStream_t s, bool fast_loc) {
   if (p1 <= p0) return hipSuccess;
   if (kp.nlev <= kColMaxLev && !getenv("LETKF_SEARCH_POINT")) {
-    hipLaunchKernelGGL((k_search_col<T, true>), dim3((unsigned)((kp.nij1 + 3) / 4)), dim3(256), 0, s, kp, tb, p0, p1,
-                       nullptr, loff, lobs, lrloc);
+    if (fast_loc)
+      hipLaunchKernelGGL((k_search_col<T, true, true>), dim3((unsigned)((kp.nij1 + 3) / 4)), dim3(256), 0, s, kp, tb,
+                         p0, p1, nullptr, loff, lobs, lrloc);
+    else
+      hipLaunchKernelGGL((k_search_col<T, true, false>), dim3((unsigned)((kp.nij1 + 3) / 4)), dim3(256), 0, s, kp,
+                         tb, p0, p1, nullptr, loff, lobs, lrloc);
     return hipGetLastError();
   }
   long long blocks = (p1 - p0 + 255) / 256;
@@ -348,9 +355,9 @@ hipError_t launch_search_fill(const KParams& kp, const Tables<T>& tb, long long 
 template hipError_t launch_search_count<float>(const KParams&, const Tables<float>&, int32_t*, hipStream_t);
 template hipError_t launch_search_count<double>(const KParams&, const Tables<double>&, int32_t*, hipStream_t);
 template hipError_t launch_search_fill<float>(const KParams&, const Tables<float>&, long long, long long,
-                                              const int64_t*, int32_t*, double*, hipStream_t);
+                                              const int64_t*, int32_t*, double*, hipStream_t, bool);
 template hipError_t launch_search_fill<double>(const KParams&, const Tables<double>&, long long, long long,
-                                               const int64_t*, int32_t*, double*, hipStream_t);
+                                               const int64_t*, int32_t*, double*, hipStream_t, bool);
 
 // ------------------------------------------------------------------ scans
 // Exclusive scans over points: loff (sum of nloc, int64) and actidx (count

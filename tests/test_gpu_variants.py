@@ -12,7 +12,9 @@ are reached through switches read at plan / launch time:
                         gather instead of reading the pre-split table;
   LETKF_E1_VALU         the FP32 reduction of the 32 / 64 buckets on the
                         vector ALUs;
-  LETKF_NO_POINT_MAJOR  the register update on the caller's arrays.
+  LETKF_NO_POINT_MAJOR  the register update on the caller's arrays;
+  LETKF_EXACT_RLOC      rloc from the FP64 exponential in precision bf16x3t
+                        (the FP32 one otherwise).
 
 Bars: FP64 analyses <= 1e-10 of the field scale and local-obs counts equal
 to the oracle's; the bf16x3t analysis-mean error <= 1e-5 (members 1e-4),
@@ -58,7 +60,7 @@ def _mean_err(a3, a3o):
 
 
 @pytest.mark.parametrize("hook", ["LETKF_SEARCH_POINT", "LETKF_GRAM_BLOCK", "LETKF_GRAM_NOSPLIT", "LETKF_E1_VALU",
-                                  "LETKF_NO_POINT_MAJOR", None])
+                                  "LETKF_NO_POINT_MAJOR", "LETKF_EXACT_RLOC", None])
 @pytest.mark.parametrize("K", [64, 100])
 def test_variant_matches_oracle(monkeypatch, hook, K):
     _say(f"case variant {hook} K={K}")
