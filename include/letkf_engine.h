@@ -152,8 +152,11 @@ typedef struct letkf_stats {
   int64_t fallback_slots;  /* K <= 128: points whose rotation log       */
                            /* overflowed (form BATCHED: whose matrix is */
                            /* not SPD or whose spectrum bound is past   */
-                           /* the rational table), solved by the LDS    */
-                           /* eigensolver and the full tile transform   */
+                           /* the rational table or, with a reduced-    */
+                           /* precision Gram, past LETKF_KAPPA_MAX --   */
+                           /* their Gram then recomputed in FP64),      */
+                           /* solved by the LDS eigensolver and the     */
+                           /* full tile transform                       */
   int64_t bucket_slots[5]; /* K <= 128: points with local obs per size  */
                            /* bucket of the register eigensolve (size   */
                            /* <= 16, 32, 48, 64, 128)                   */

@@ -557,6 +557,18 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     HIP_TRY(h->xpm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
     HIP_TRY(h->apm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
   }
+  // a reduced-precision Gram resolves the weights only while the spectrum
+  // bound of the slot's tridiagonal matrix stays within kmax of beta (the
+  // analysis error grows with it, ~1e-8 of the field scale per unit with the
+  // BF16x3 Gram, ~1e-9 with the FP32 ones); wider slots go to the fallback
+  // on an FP64 Gram.  LETKF_KAPPA_MAX overrides.
+  pl.kmax = p.precision == LETKF_PREC_FP64 ? 1e308
+            : (p.precision == LETKF_PREC_BF16X3 || p.precision == LETKF_PREC_BF16X3_TRI32) ? 1024.0
+                                                                                              : 8192.0;
+  if (const char* env = getenv("LETKF_KAPPA_MAX")) {
+    const double v = atof(env);
+    if (v > 1.0) pl.kmax = v;
+  }
   // rloc from the FP32 exponential in precision bf16x3t (test hook
   // LETKF_EXACT_RLOC: the FP64 one)
   pl.fastloc = p.precision == LETKF_PREC_BF16X3_TRI32 && !getenv("LETKF_EXACT_RLOC");
@@ -669,7 +681,7 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
                                         h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
                                         h->status.as<int>(), h->vpath[q],
                                         pl.rat ? h->ratt.as<double>() : nullptr,
-                                        p.precision == LETKF_PREC_BF16X3_TRI32, ktm, pm, s));
+                                        p.precision == LETKF_PREC_BF16X3_TRI32, ktm, pm, pl.kmax, s));
         }
       } else {
         HIP_TRY(launch_eig3(K, na, h->G.as<double>(), h->V.as<double>(), h->eD.as<double>(), h->eE.as<double>(),
@@ -1081,7 +1093,7 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
                                           h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(),
                                           h->lrloc.as<double>(), h->yT.as<double>(), h->bvec.as<double>(),
                                           h->status.as<int>(), v, pl.rat ? h->ratt.as<double>() : nullptr,
-                                          p.precision == LETKF_PREC_BF16X3_TRI32, nullptr, pm, s));
+                                          p.precision == LETKF_PREC_BF16X3_TRI32, nullptr, pm, pl.kmax, s));
             HIP_TRY(hipEventRecord(e1, s));
             HIP_TRY(hipEventSynchronize(e1));
             float t = 0.f;

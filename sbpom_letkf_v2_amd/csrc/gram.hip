@@ -6,6 +6,7 @@
 //       wvec = Pa (Rlinv_dYf^T dep), an exact re-association of :162-184)
 // One workgroup per local problem; its rows are staged through LDS in panels
 // of 2048/KT rows.
+#include <algorithm>
 #include <cstdlib>
 
 #include "letkf_internal.h"
@@ -228,10 +229,11 @@ __global__ void __launch_bounds__(256) k_gram_fp64(int K, double diag, Src src, 
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
 constexpr int kP64 = 16;  // summation rows per LDS panel
 
+// One slot sb of the FP64 products on the whole workgroup (every thread
+// calls it, with the same sb).
 template <int OUT, typename Src>
-__global__ void __launch_bounds__(256) k_gram_mfma64(int K, double diag, Src src, double* __restrict__ G,
-                                                     double* __restrict__ bvec, int32_t* __restrict__ nrow,
-                                                     int dual) {
+__device__ __forceinline__ void gram64_slot(int sb, int K, double diag, Src src, double* __restrict__ G,
+                                            double* __restrict__ bvec, int32_t* __restrict__ nrow, int dual) {
   constexpr int T = OUT / 16;
   constexpr int NU = T * (T + 1) / 2;
   constexpr int NT = (NU + 3) / 4;
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma64(int K, double diag, Src src
   __shared__ double ds[kP64];
   __shared__ double sws[OUT];
   __shared__ int ios[OUT];
-  const int n = src.begin(blockIdx.x);
+  const int n = src.begin(sb);
   const bool dm = dual && n < K;
   if (dual == 2 && !dm) return;  // dual slots only (the FP32 MFMA kernel takes the others)
   const int t = threadIdx.x, lane = t & 63;
@@ -274,7 +276,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma64(int K, double diag, Src src
       src.brow(l, io, sw);
       ios[l] = io;
       sws[l] = sw;
-      bvec[(long long)blockIdx.x * K + l] = sw * src.dep(l);
+      bvec[(long long)sb * K + l] = sw * src.dep(l);
     }
     __syncthreads();
   }
@@ -335,7 +337,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma64(int K, double diag, Src src
       for (int l = 0; l < ps; ++l) bacc = fma(As[l][t], ds[l], bacc);
     __syncthreads();
   }
-  double* Gp = G + (long long)blockIdx.x * K * K;
+  double* Gp = G + (long long)sb * K * K;
 #pragma unroll
   for (int q = 0; q < NT; ++q) {
     if (!live[q]) continue;
@@ -347,8 +349,43 @@ __global__ void __launch_bounds__(256) k_gram_mfma64(int K, double diag, Src src
         Gp[row * N + col] = acc[q][r] + (!dm && row == col ? diag : 0.0);
     }
   }
-  if (!dm && t < K) bvec[(long long)blockIdx.x * K + t] = bacc;
-  if (t == 0 && nrow) nrow[blockIdx.x] = n;
+  if (!dm && t < K) bvec[(long long)sb * K + t] = bacc;
+  if (t == 0 && nrow) nrow[sb] = n;
+}
+
+template <int OUT, typename Src>
+__global__ void __launch_bounds__(256) k_gram_mfma64(int K, double diag, Src src, double* __restrict__ G,
+                                                     double* __restrict__ bvec, int32_t* __restrict__ nrow,
+                                                     int dual) {
+  gram64_slot<OUT, Src>(blockIdx.x, K, diag, src, G, bvec, nrow, dual);
+}
+
+// The slots list[0..cnt) with flags[slot] == -1 (those the mixed-precision
+// update left to the fallback): each workgroup tests 256 positions a pass
+// and runs the products of the flagged ones.
+template <int OUT, typename Src>
+__global__ void __launch_bounds__(256) k_gram_mfma64_flagged(int K, double diag, Src src, double* __restrict__ G,
+                                                             double* __restrict__ bvec, int dual,
+                                                             const int32_t* __restrict__ list, int cnt,
+                                                             const int32_t* __restrict__ flags) {
+  __shared__ unsigned long long fm[4];
+  const int t = threadIdx.x;
+  for (int base = blockIdx.x * 256; base < cnt; base += gridDim.x * 256) {
+    const int pos = base + t;
+    const bool f = pos < cnt && flags[list[pos]] == -1;
+    const unsigned long long m = __ballot(f);
+    if ((t & 63) == 0) fm[t >> 6] = m;
+    __syncthreads();
+    for (int wq = 0; wq < 4; ++wq) {
+      unsigned long long mm = fm[wq];
+      while (mm) {
+        const int bit = __ffsll((long long)mm) - 1;
+        mm &= mm - 1;
+        gram64_slot<OUT, Src>(list[base + 64 * wq + bit], K, diag, src, G, bvec, nullptr, dual);
+      }
+    }
+    __syncthreads();  // the next pass rewrites fm
+  }
 }
 
 template <typename Src>
@@ -372,6 +409,32 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb, const double
   const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
   return launch_gram_src(kp.K, diag, src, na, G, bvec, nullptr, dual, s);
 }
+
+template <typename T>
+hipError_t launch_gram_fp64_flagged(const KParams& kp, const Tables<T>& tb, const double* yT, const int32_t* act_p,
+                                    const int64_t* loff, long long lbase, const int32_t* lobs, const double* lrloc,
+                                    int a0, const int32_t* list, int cnt, const int32_t* flags, double* G,
+                                    double* bvec, int dual, hipStream_t s) {
+  if (cnt <= 0 || kp.K > 128) return hipSuccess;
+  ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0};
+  const double diag = (double)((T)(kp.K - 1) / (T)kp.rho);
+  const unsigned grid = (unsigned)std::min((cnt + 255) / 256, 1024);
+  if (kp.K <= 64)
+    hipLaunchKernelGGL((k_gram_mfma64_flagged<64, ObsSource<T>>), dim3(grid), dim3(256), 0, s, kp.K, diag, src, G,
+                       bvec, dual ? 1 : 0, list, cnt, flags);
+  else
+    hipLaunchKernelGGL((k_gram_mfma64_flagged<128, ObsSource<T>>), dim3(grid), dim3(256), 0, s, kp.K, diag, src, G,
+                       bvec, dual ? 1 : 0, list, cnt, flags);
+  return hipGetLastError();
+}
+template hipError_t launch_gram_fp64_flagged<float>(const KParams&, const Tables<float>&, const double*,
+                                                    const int32_t*, const int64_t*, long long, const int32_t*,
+                                                    const double*, int, const int32_t*, int, const int32_t*,
+                                                    double*, double*, int, hipStream_t);
+template hipError_t launch_gram_fp64_flagged<double>(const KParams&, const Tables<double>&, const double*,
+                                                     const int32_t*, const int64_t*, long long, const int32_t*,
+                                                     const double*, int, const int32_t*, int, const int32_t*,
+                                                     double*, double*, int, hipStream_t);
 
 template <typename T>
 hipError_t launch_gram_csr(int K, double rho, int nb, const int64_t* offs, const T* hdxf, const T* rdiag,

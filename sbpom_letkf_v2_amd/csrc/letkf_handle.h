@@ -62,6 +62,7 @@ struct AnalysisPlan {
   int pm_ld = 0;     // register update through point-major chunk copies: their row pitch (0: none)
   bool ysplit = false;  // the dual-form BF16x3 Gram reads the obs rows pre-split (ysp)
   bool fastloc = false; // rloc from the FP32 exponential (precision bf16x3t)
+  double kmax = 1e308;  // spectrum width past which a slot takes the FP64 fallback
   std::vector<long long> bpts, bv;  // chunk point bounds; per-chunk (local obs, active slot) offsets + totals
   std::vector<int32_t> hist, boff;  // per (chunk, size bucket) slot counts and list offsets
   letkf_stats_t st{};               // counters known from the plan

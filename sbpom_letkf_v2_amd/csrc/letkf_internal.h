@@ -102,6 +102,15 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb,
 __host__ __device__ inline int split_obs_groups(int K) { return 2 * ((K + 15) / 16); }
 hipError_t launch_split_obs(int nobs, int K, const double* yT, void* ysp, hipStream_t s);
 
+// The FP64 products again for the slots list[0..cnt) with flags[slot] == -1
+// (a size bucket's slots that the mixed-precision update left to the
+// fallback), so that the fallback solves them from an FP64 Gram.
+template <typename T>
+hipError_t launch_gram_fp64_flagged(const KParams& kp, const Tables<T>& tb, const double* yT, const int32_t* act_p,
+                                    const int64_t* loff, long long lbase, const int32_t* lobs, const double* lrloc,
+                                    int a0, const int32_t* list, int cnt, const int32_t* flags, double* G,
+                                    double* bvec, int dual, hipStream_t s);
+
 // Same products for letkf_core_batched: problem b owns rows offs[b]..offs[b+1)
 // of hdxf (K values per row); nrow[b] receives the row count.
 template <typename T>
@@ -226,7 +235,7 @@ hipError_t launch_eigr_update(int nb, int K, int dual, const int32_t* nloc,
                               const int32_t* lobs, const double* lrloc,
                               const double* yT, const double* bvec,
                               int* status, int vpath, const double* rat, bool tri32, EigTimer* tm,
-                              const PointMajor<T>& pm, hipStream_t s);
+                              const PointMajor<T>& pm, double kmax, hipStream_t s);
 // The rational path's coefficient table (ratfun.inc), rat_table_size()
 // doubles, built once per process; rat == nullptr above selects E2 + V2.
 // Layout: [kind][class][n, d, a[kRatTablePoles], s[kRatTablePoles]], kinds

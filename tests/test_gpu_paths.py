@@ -283,3 +283,25 @@ def test_degenerate_obs_ensemble(precision):
         m, mo = a3.mean(axis=2), a3o.mean(axis=2)
         em = max(float(np.abs(m[..., v] - mo[..., v]).max() / np.abs(mo[..., v]).max()) for v in range(a3.shape[3]))
         assert em <= 1e-5
+
+
+@pytest.mark.parametrize("scale,kmax,bar", [(300.0, None, 1e-7), (10.0, "256", 1e-5)])
+def test_wide_spectra_promoted_to_fp64(monkeypatch, scale, kmax, bar):
+    """A third of the obs with perturbations `scale` times the others: the
+    local matrices' spectra widen past what the BF16x3 Gram resolves.  Slots
+    whose spectrum bound exceeds the limit (LETKF_KAPPA_MAX, default 1024 for
+    bf16x3t) take the fallback on an FP64 Gram, and the analysis stays within
+    `bar` of the FP64 oracle (analysis mean, relative to the field scale)."""
+    _say(f"case wide spectra scale={scale} kmax={kmax}")
+    if kmax:
+        monkeypatch.setenv("LETKF_KAPPA_MAX", kmax)
+    pr = synth.make_problem(nlon=24, nlat=20, nlev=5, nmem=40, nobs=600, seed=131, alpha_rtpp=0.5,
+                            real_kind=8, sigma_h_cells=1.5)
+    pr.obshdxf[:, 0::3] *= scale   # (nmem, nobs): the Fortran (nobs, nmem) array
+    a3o, a2o, nl = _oracle(pr)
+    a3, a2, st = _engine(pr, "bf16x3t", "batched")
+    m, mo = a3.mean(axis=2), a3o.mean(axis=2)
+    em = max(float(np.abs(m[..., v] - mo[..., v]).max() / np.abs(mo[..., v]).max()) for v in range(a3.shape[3]))
+    _say(f"  mean err={em:.2e} fallback slots={st['fallback_slots']}")
+    assert st["fallback_slots"] > 0
+    assert em <= bar

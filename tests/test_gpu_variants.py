@@ -10,8 +10,9 @@ are reached through switches read at plan / launch time:
   LETKF_GRAM_BLOCK      the BF16x3 Gram on the four-wavefront block kernel;
   LETKF_GRAM_NOSPLIT    the dual-form BF16x3 Gram splitting the FP64 rows per
                         gather instead of reading the pre-split table;
-  LETKF_E1_VALU         the FP32 reduction of the 32 / 64 buckets on the
-                        vector ALUs;
+  LETKF_E1_VALU         the FP32 reduction of the 16 bucket one matrix a
+                        wavefront and of the 32 / 64 buckets on the vector
+                        ALUs;
   LETKF_NO_POINT_MAJOR  the register update on the caller's arrays;
   LETKF_EXACT_RLOC      rloc from the FP64 exponential in precision bf16x3t
                         (the FP32 one otherwise).
