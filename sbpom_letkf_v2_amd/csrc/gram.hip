@@ -731,8 +731,12 @@ __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int 
   const bool two = n > 32;  // tiles (0,0), (0,1), (1,1); else (0,0) only
   mf_v16 acc00 = mf_v16{}, acc01 = mf_v16{}, acc11 = mf_v16{};
   // x -> (hi, lo) in element e of two operand vectors
+  // x -> FP32 -> BF16 hi + lo, each step rounded to nearest (the empty asm
+  // keeps the compiler from fusing the two conversions of hi into a
+  // correctly rounded FP64 -> BF16 one, a dozen instructions more)
   auto split = [](double x, mf_bf8& hi, mf_bf8& lo, int e) {
-    const float v = (float)x;
+    float v = (float)x;
+    asm volatile("" : "+v"(v));
     const __bf16 vh = (__bf16)v;
     hi[e] = vh;
     lo[e] = (__bf16)(v - (float)vh);
@@ -872,8 +876,12 @@ __global__ void __launch_bounds__(256) k_gram_wave_kk(int K, double diag, ObsSou
   const bool ok0 = c < K, ok1 = 32 + c < K;
   mf_v16 acc00 = mf_v16{}, acc01 = mf_v16{}, acc11 = mf_v16{};
   double bp0 = 0.0, bp1 = 0.0;
+  // x -> FP32 -> BF16 hi + lo, each step rounded to nearest (the empty asm
+  // keeps the compiler from fusing the two conversions of hi into a
+  // correctly rounded FP64 -> BF16 one, a dozen instructions more)
   auto split = [](double x, mf_bf8& hi, mf_bf8& lo, int e) {
-    const float v = (float)x;
+    float v = (float)x;
+    asm volatile("" : "+v"(v));
     const __bf16 vh = (__bf16)v;
     hi[e] = vh;
     lo[e] = (__bf16)(v - (float)vh);
@@ -900,14 +908,16 @@ __global__ void __launch_bounds__(256) k_gram_wave_kk(int K, double diag, ObsSou
         y0[e] = yr[m0];
         y1[e] = two ? yr[m1] : 0.0;
       }
+      // (a lane past K holds a clamped member: it feeds only rows and
+      // columns >= K of the tiles and its own b, none of which is stored)
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const RowMeta rm = meta[w][l0 + 8 * h + e];
-        const double v0 = ok0 ? rm.sw * y0[e] : 0.0;
+        const double v0 = rm.sw * y0[e];
         bp0 = fma(v0, rm.swd, bp0);
         split(v0, x0h, x0l, e);
         if (two) {
-          const double v1 = ok1 ? rm.sw * y1[e] : 0.0;
+          const double v1 = rm.sw * y1[e];
           bp1 = fma(v1, rm.swd, bp1);
           split(v1, x1h, x1l, e);
         }
