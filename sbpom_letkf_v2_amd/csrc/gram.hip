@@ -721,7 +721,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
 template <typename T, bool SPLIT>
 __global__ void __launch_bounds__(256) k_gram_wave(int K, ObsSource<T> src, int na, double* __restrict__ G,
                                                    double* __restrict__ bvec) {
-  __shared__ double swl[SPLIT ? 4 : 1][64];  // SPLIT: the rows' sqrt weights
+  __shared__ double swl[4][64];  // SPLIT: the rows' sqrt weights
   const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int b = blockIdx.x * 4 + w;
