@@ -556,6 +556,10 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     pl.pm_ld = K * (p.nv3d + p.nv2d);
     HIP_TRY(h->xpm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
     HIP_TRY(h->apm.ensure(sizeof(T) * (size_t)PC * pl.pm_ld));
+    // and the forecast statistics the copy forms (nmem <= 64; test hook
+    // LETKF_NO_PM_STATS: the vector kernels form them)
+    pl.pm_stats = K <= 64 && !getenv("LETKF_NO_PM_STATS");
+    if (pl.pm_stats) HIP_TRY(h->pst.ensure(sizeof(double2) * (size_t)PC * (p.nv3d + p.nv2d)));
   }
   // a reduced-precision Gram resolves the weights only while the spectrum
   // bound of the slot's tridiagonal matrix stays within kmax of beta (the
@@ -664,7 +668,9 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
     if (na > 0 && reg_eig && pl.pm_ld > 0) {
       pm.X = h->xpm.as<T>();
       pm.A = h->apm.as<T>();
-      HIP_TRY(launch_to_point_major<T>(kp, p0, p1, f3, f2, h->xpm.as<T>(), pl.pm_ld, s));
+      double2* st = pl.pm_stats ? h->pst.as<double2>() : nullptr;
+      pm.S = st;
+      HIP_TRY(launch_to_point_major<T>(kp, p0, p1, f3, f2, h->xpm.as<T>(), pl.pm_ld, s, st));
     }
     if (na > 0) {
       if (reg_eig) {
@@ -1071,7 +1077,9 @@ static int autotune_device(letkf_handle* h, const letkf_grid_t* g, const letkf_o
       if (pl.pm_ld > 0) {
         pm.X = h->xpm.as<T>();
         pm.A = h->apm.as<T>();
-        HIP_TRY(launch_to_point_major<T>(kp, pl.bpts[0], pl.bpts[1], f3, f2, h->xpm.as<T>(), pl.pm_ld, s));
+        double2* st = pl.pm_stats ? h->pst.as<double2>() : nullptr;
+        pm.S = st;
+        HIP_TRY(launch_to_point_major<T>(kp, pl.bpts[0], pl.bpts[1], f3, f2, h->xpm.as<T>(), pl.pm_ld, s, st));
       }
       hipEvent_t e0, e1;
       HIP_TRY(hipEventCreate(&e0));

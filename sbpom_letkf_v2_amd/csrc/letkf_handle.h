@@ -60,6 +60,7 @@ struct AnalysisPlan {
   int dual = 0;
   bool rat = false;  // f(T) by rational functions (batched form, K <= 128)
   int pm_ld = 0;     // register update through point-major chunk copies: their row pitch (0: none)
+  bool pm_stats = false;  // the copies also form each point's forecast mean and spread (pst)
   bool ysplit = false;  // the dual-form BF16x3 Gram reads the obs rows pre-split (ysp)
   bool fastloc = false; // rloc from the FP32 exponential (precision bf16x3t)
   double kmax = 1e308;  // spectrum width past which a slot takes the FP64 fallback
@@ -99,6 +100,7 @@ struct letkf_handle {
   DevBuf adj_pair, adj_sort;  // adjoint: per-pair terms; sort keys, values and scratch
   DevBuf bhist, bcur, blist;              // per-chunk size buckets of active slots
   DevBuf xpm, apm;                        // point-major forecast / analysis of a chunk (register update)
+  DevBuf pst;                             // their forecast (mean, spread) per point and variable
   DevBuf ysp;                             // obs rows as BF16 hi / lo groups (dual-form BF16x3 Gram)
   // staging for the host-pointer entry point
   DevBuf h_grid, h_obs, h_f3, h_f2, h_a3, h_a2;
@@ -126,7 +128,7 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &ratt, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &xpm, &apm, &ysp, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &ratt, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &xpm, &apm, &pst, &ysp, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
     ratt_ok = false;
     for (hipEvent_t e : events) (void)hipEventDestroy(e);

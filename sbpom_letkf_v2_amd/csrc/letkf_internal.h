@@ -210,10 +210,14 @@ struct PointMajor {
   T* A;
   long long p0;
   int ld;
+  const double2* S = nullptr;  // forecast (mean, spread) of each (chunk point, variable), or none
 };
+// S (nmem <= 64, optional): the forecast mean and spread of every (point,
+// variable) of the chunk, S[(p - p0) (nv3d + nv2d) + v], formed from the
+// tile the copy holds (the variables a point does not have are left).
 template <typename T>
 hipError_t launch_to_point_major(const KParams& kp, long long p0, long long p1, const T* X3, const T* X2,
-                                 T* Xp, int ld, hipStream_t s);
+                                 T* Xp, int ld, hipStream_t s, double2* S = nullptr);
 template <typename T>
 hipError_t launch_from_point_major(const KParams& kp, long long p0, long long p1, const T* Ap, int ld,
                                    const int32_t* nloc, T* A3, T* A2, hipStream_t s);

@@ -471,7 +471,8 @@ __global__ void __launch_bounds__(256) k_transform(KParams kp, Tables<T> tb, lon
 // by k_noobs, and the fallback slots by the tile transform after this).
 template <typename T>
 __global__ void __launch_bounds__(256) k_to_point_major(KParams kp, long long p0, long long p1, const T* __restrict__ X3,
-                                                        const T* __restrict__ X2, T* __restrict__ Xp, int ld) {
+                                                        const T* __restrict__ X2, T* __restrict__ Xp, int ld,
+                                                        double2* __restrict__ S) {
   __shared__ T tile[64][65];
   const int v = blockIdx.z, K = kp.K, nv3 = kp.nv3d;
   const long long pb = p0 + (long long)blockIdx.x * 64;
@@ -502,6 +503,32 @@ __global__ void __launch_bounds__(256) k_to_point_major(KParams kp, long long p0
     const long long p = pb + r;
     const int m = mb + tx;
     if (p < p1 && m < K && (v < nv3 || p >= surf0)) Xp[(p - p0) * ld + (long long)K * v + m] = tile[tx][r];
+  }
+  if (S != nullptr && K <= 64) {
+    // the forecast mean and spread of point tx: partial sums over the
+    // members ty + 4 k, then the four partials
+    __shared__ double red[4][64];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (ty + 4 * k < K) s += (double)tile[ty + 4 * k][tx];
+    red[ty][tx] = s;
+    __syncthreads();
+    const double fm = ((red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx])) * (1.0 / (double)K);
+    __syncthreads();
+    double s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (ty + 4 * k < K) {
+        const double d = (double)tile[ty + 4 * k][tx] - fm;
+        s2 = fma(d, d, s2);
+      }
+    red[ty][tx] = s2;
+    __syncthreads();
+    const long long p = pb + tx;
+    if (ty == 0 && p < p1 && (v < nv3 || p >= surf0))
+      S[(p - p0) * (nv3 + kp.nv2d) + v] =
+          make_double2(fm, sqrt(((red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx])) / (double)(K - 1)));
   }
 }
 
@@ -544,10 +571,10 @@ __global__ void __launch_bounds__(256) k_from_point_major(KParams kp, long long 
 
 template <typename T>
 hipError_t launch_to_point_major(const KParams& kp, long long p0, long long p1, const T* X3, const T* X2, T* Xp,
-                                 int ld, hipStream_t s) {
+                                 int ld, hipStream_t s, double2* S) {
   if (p1 <= p0) return hipSuccess;
   const dim3 grid((unsigned)((p1 - p0 + 63) / 64), (unsigned)((kp.K + 63) / 64), (unsigned)(kp.nv3d + kp.nv2d));
-  hipLaunchKernelGGL(k_to_point_major<T>, grid, dim3(256), 0, s, kp, p0, p1, X3, X2, Xp, ld);
+  hipLaunchKernelGGL(k_to_point_major<T>, grid, dim3(256), 0, s, kp, p0, p1, X3, X2, Xp, ld, S);
   return hipGetLastError();
 }
 
@@ -560,9 +587,9 @@ hipError_t launch_from_point_major(const KParams& kp, long long p0, long long p1
   return hipGetLastError();
 }
 template hipError_t launch_to_point_major<float>(const KParams&, long long, long long, const float*, const float*,
-                                                 float*, int, hipStream_t);
+                                                 float*, int, hipStream_t, double2*);
 template hipError_t launch_to_point_major<double>(const KParams&, long long, long long, const double*,
-                                                  const double*, double*, int, hipStream_t);
+                                                  const double*, double*, int, hipStream_t, double2*);
 template hipError_t launch_from_point_major<float>(const KParams&, long long, long long, const float*, int,
                                                    const int32_t*, float*, float*, hipStream_t);
 template hipError_t launch_from_point_major<double>(const KParams&, long long, long long, const double*, int,

@@ -15,7 +15,9 @@ are reached through switches read at plan / launch time:
                         ALUs;
   LETKF_NO_POINT_MAJOR  the register update on the caller's arrays;
   LETKF_EXACT_RLOC      rloc from the FP64 exponential in precision bf16x3t
-                        (the FP32 one otherwise).
+                        (the FP32 one otherwise);
+  LETKF_NO_PM_STATS     the forecast mean and spread formed by the vector
+                        kernels instead of the point-major copy.
 
 Bars: FP64 analyses <= 1e-10 of the field scale and local-obs counts equal
 to the oracle's; the bf16x3t analysis-mean error <= 1e-5 (members 1e-4),
@@ -61,7 +63,7 @@ def _mean_err(a3, a3o):
 
 
 @pytest.mark.parametrize("hook", ["LETKF_SEARCH_POINT", "LETKF_GRAM_BLOCK", "LETKF_GRAM_NOSPLIT", "LETKF_E1_VALU",
-                                  "LETKF_NO_POINT_MAJOR", "LETKF_EXACT_RLOC", None])
+                                  "LETKF_NO_POINT_MAJOR", "LETKF_EXACT_RLOC", "LETKF_NO_PM_STATS", None])
 @pytest.mark.parametrize("K", [64, 100])
 def test_variant_matches_oracle(monkeypatch, hook, K):
     _say(f"case variant {hook} K={K}")
