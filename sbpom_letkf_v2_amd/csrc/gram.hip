@@ -465,7 +465,10 @@ template hipError_t launch_gram_fp64<double>(const KParams&, const Tables<double
 // four wavefronts (kTile), tiles past the problem skipped:
 //   prec FP32   : v_mfma_f32_32x32x2f32 on FP32-rounded operands;
 //   prec BF16X3 : operands split x = hi + lo (bf16 each), three products
-//                 hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_bf16.
+//                 hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_bf16; the
+//                 split is done once as a panel is staged, and the primal
+//                 rows are scaled by sqrt(rloc / rdiag) so that one panel
+//                 serves as both operands (as in the dual form).
 // The innovation term b (and dt in the dual form) stays FP64; the result is
 // widened to FP64 for the eigensolve.  prec FP32_PANEL accumulates in two
 // levels: the MFMA accumulators sum one panel of 32 obs or members and are
@@ -507,11 +510,21 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   // Gram waiting a memory latency per element; 2 / 4 / 8 measured 121 / 103
   // / 129 ms at c2, the last at occupancy 3)
   constexpr int kBatch = OUT == 128 ? (kSplit ? 4 : 8) : 4;
+  // BF16X3: one operand panel for both sides (the primal rows scaled by the
+  // square root of their weight, as the dual rows are), split into BF16 hi
+  // and lo once at staging and stored transposed -- rows = output index,
+  // kPanel summation values + 8 of padding (80 bytes: 16-byte aligned rows)
+  // -- so each lane loads its eight operand values with one 16-byte read
+  constexpr bool kBf = PREC == LETKF_PREC_BF16X3;
+  constexpr int PB = kPanel + 8;
   // panel: rows = summation index (obs or member), cols = output index
-  __shared__ float As[kPanel][LP];
-  __shared__ float Bs[kPanel][LP];
+  __shared__ float As[kBf ? 1 : kPanel][LP];
+  __shared__ float Bs[kBf ? 1 : kPanel][LP];
   __shared__ float Al[kLoRows][LP];  // FP32X3: x - (double)hi rounded to FP32
   __shared__ float Bl[kLoRows][LP];
+  __shared__ __align__(16) __bf16 Sh[kBf ? OUT : 1][PB];
+  __shared__ __align__(16) __bf16 Sl[kBf ? OUT : 1][PB];
+  __shared__ double sqw[kBf ? kPanel : 1];  // BF16X3 primal: sqrt weight per panel row
   __shared__ double sws[OUT];  // dual: sqrt weight per obs; primal: weight per panel row
   __shared__ double dps[kPanel];
   __shared__ int ios[OUT];
@@ -519,6 +532,15 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   const int n = src.begin(blockIdx.x);
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: tile tables in scalar registers
+  // BF16X3 staging: x -> FP32 -> BF16 hi + BF16 (x32 - hi), each step rounded
+  // (the empty asm keeps the two conversions from being fused into one)
+  auto put_bf = [&](int row, int col, double x) {
+    float v = (float)x;
+    asm volatile("" : "+v"(v));
+    const __bf16 h = (__bf16)v;
+    Sh[col][row] = h;
+    Sl[col][row] = (__bf16)(v - (float)h);
+  };
   const bool dm = dual && n < K;
   if (dual == 2 && dm) return;  // K x K slots only (the FP64 kernel takes the dual ones)
   if (dual == 3 && dm && n <= 64) return;  // dual slots up to 64 rows: k_gram_wave
@@ -572,6 +594,10 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
         for (int u = 0; u < kBatch; ++u) {
           const int q = t + 256 * (u0 + u), row = q & (kPanel - 1), col = q / kPanel;
           const double x = col < N ? sws[col] * yv[u] : 0.0;
+          if constexpr (kBf) {
+            put_bf(row, col, x);
+            continue;
+          }
           const float v = (float)x;
           As[row][col] = v;
           Bs[row][col] = v;
@@ -590,6 +616,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
         ios[t] = io;
         sws[t] = w;
         dps[t] = d;
+        if (kBf) sqw[t] = sqrt(w);
       }
       __syncthreads();
       // Y panel: lanes walk the members of one obs, batches of kBatch loads
@@ -606,6 +633,15 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
         for (int u = 0; u < kBatch; ++u) {
           const int q = t + 256 * (u0 + u), row = q / OUT, col = q % OUT;
           float av = 0.f, bv = 0.f, al = 0.f, bl = 0.f;
+          if constexpr (kBf) {
+            double xs = 0.0;
+            if (row < ps && col < N) {
+              bpart = fma(yv[u] * sws[row], dps[row], bpart);
+              xs = yv[u] * sqw[row];
+            }
+            put_bf(row, col, xs);
+            continue;
+          }
           if (row < ps && col < N) {
             const double wy = yv[u] * sws[row];
             bpart = fma(wy, dps[row], bpart);
@@ -626,7 +662,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
       }
     }
     __syncthreads();
-    if (PREC != LETKF_PREC_BF16X3) {
+    if constexpr (PREC != LETKF_PREC_BF16X3) {
 #pragma unroll 2
       for (int k = 0; k < ps; k += 2) {
         const int kk = k + (lane >> 5);
@@ -645,22 +681,17 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
         }
       }
     } else {
+      // summation values k .. k + 15 (zero past ps: the staging fills the panel)
       for (int k = 0; k < ps; k += 16) {
+        const int ko = k + 8 * (lane >> 5);
 #pragma unroll
         for (int q = 0; q < NT; ++q) {
           if (!live[q]) continue;
-          mf_bf8 ah, al, bh, bl;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int kk = k + 8 * (lane >> 5) + e;
-            const float x = kk < ps ? As[kk][tr[q] + (lane & 31)] : 0.f;
-            const float y = kk < ps ? Bs[kk][tc[q] + (lane & 31)] : 0.f;
-            const __bf16 x1 = (__bf16)x, y1 = (__bf16)y;
-            ah[e] = x1;
-            bh[e] = y1;
-            al[e] = (__bf16)(x - (float)x1);
-            bl[e] = (__bf16)(y - (float)y1);
-          }
+          const int ra = tr[q] + (lane & 31), rb = tc[q] + (lane & 31);
+          const mf_bf8 ah = *reinterpret_cast<const mf_bf8*>(&Sh[ra][ko]);
+          const mf_bf8 al = *reinterpret_cast<const mf_bf8*>(&Sl[ra][ko]);
+          const mf_bf8 bh = *reinterpret_cast<const mf_bf8*>(&Sh[rb][ko]);
+          const mf_bf8 bl = *reinterpret_cast<const mf_bf8*>(&Sl[rb][ko]);
           acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[q], 0, 0, 0);
           acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[q], 0, 0, 0);
           acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[q], 0, 0, 0);
