@@ -496,7 +496,8 @@ __constant__ int kTile128[4][3] = {{0 * 4 + 0, 0 * 4 + 1, 0 * 4 + 2},
 
 template <int PREC, int OUT, typename Src>
 __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, double* __restrict__ G,
-                                                   double* __restrict__ bvec, int dual) {
+                                                   double* __restrict__ bvec, int dual,
+                                                   const int32_t* __restrict__ list) {
   constexpr int NT = OUT == 128 ? 3 : 1;          // tiles per wavefront
   constexpr bool kTwoLevel = PREC == LETKF_PREC_FP32_PANEL;
   constexpr bool kSplit = PREC == LETKF_PREC_FP32X3;  // operands as FP32 hi + lo
@@ -529,7 +530,9 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
   __shared__ double dps[kPanel];
   __shared__ int ios[OUT];
   __shared__ double bred[256];
-  const int n = src.begin(blockIdx.x);
+  // list (optional): the slots this launch covers, one workgroup each
+  const int b = list ? list[blockIdx.x] : (int)blockIdx.x;
+  const int n = src.begin(b);
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: tile tables in scalar registers
   // BF16X3 staging: x -> FP32 -> BF16 hi + BF16 (x32 - hi), each step rounded
@@ -569,7 +572,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
       src.brow(l, io, sw);
       ios[l] = io;
       sws[l] = sw;
-      bvec[(long long)blockIdx.x * K + l] = sw * src.dep(l);
+      bvec[(long long)b * K + l] = sw * src.dep(l);
     }
     __syncthreads();
   }
@@ -714,10 +717,10 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int K, double diag, Src src, 
       double bs = 0.0;
 #pragma unroll
       for (int r = 0; r < RS; ++r) bs += bred[t + r * OUT];
-      bvec[(long long)blockIdx.x * K + t] = bs;
+      bvec[(long long)b * K + t] = bs;
     }
   }
-  double* Gp = G + (long long)blockIdx.x * K * K;
+  double* Gp = G + (long long)b * K * K;
 #pragma unroll
   for (int q = 0; q < NT; ++q) {
     if (!live[q]) continue;
@@ -993,20 +996,20 @@ __global__ void __launch_bounds__(256) k_gram_wave_kk(int K, double diag, ObsSou
 
 template <int PREC, typename T>
 static void launch_gram_mixed_p(int K, double diag, const ObsSource<T>& src, int na, double* G, double* bvec,
-                                int dual, hipStream_t s) {
+                                int dual, hipStream_t s, const int32_t* list = nullptr) {
   if (K <= 64)
     hipLaunchKernelGGL((k_gram_mfma<PREC, 64, ObsSource<T>>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec,
-                       dual);
+                       dual, list);
   else
     hipLaunchKernelGGL((k_gram_mfma<PREC, 128, ObsSource<T>>), dim3(na), dim3(256), 0, s, K, diag, src, G, bvec,
-                       dual);
+                       dual, list);
 }
 
 template <typename T>
 hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, const double* yT,
                              const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
                              const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
-                             const void* ysp, hipStream_t s) {
+                             const void* ysp, hipStream_t s, const int32_t* list128, int cnt128) {
   if (na <= 0) return hipSuccess;
   if (kp.K > 128) return hipErrorInvalidValue;
   ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0, static_cast<const mf_bf8*>(ysp)};
@@ -1025,6 +1028,12 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
       return hipGetLastError();
     }
     if (dual) dual = 3;
+    // the rest are the slots of the 128 bucket: launched over its list when given
+    if (dual && list128) {
+      if (cnt128 > 0)
+        launch_gram_mixed_p<LETKF_PREC_BF16X3, T>(kp.K, diag, src, cnt128, G, bvec, dual, s, list128);
+      return hipGetLastError();
+    }
   }
   if (prec == LETKF_PREC_FP32)
     launch_gram_mixed_p<LETKF_PREC_FP32, T>(kp.K, diag, src, na, G, bvec, dual, s);
@@ -1039,10 +1048,10 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
 template hipError_t launch_gram_mixed<float>(int, const KParams&, const Tables<float>&, const double*,
                                              const int32_t*, const int64_t*, long long, const int32_t*,
                                              const double*, int, int, double*, double*, int, const void*,
-                                             hipStream_t);
+                                             hipStream_t, const int32_t*, int);
 template hipError_t launch_gram_mixed<double>(int, const KParams&, const Tables<double>&, const double*,
                                               const int32_t*, const int64_t*, long long, const int32_t*,
                                               const double*, int, int, double*, double*, int, const void*,
-                                              hipStream_t);
+                                              hipStream_t, const int32_t*, int);
 
 }  // namespace letkf
