@@ -328,8 +328,8 @@ template <typename T>
 static hipError_t launch_gram_any(int prec, const KParams& kp, const Tables<T>& tb, const double* yT,
                                   const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
                                   const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
-                                  const void* ysp, hipStream_t s, const int32_t* list128 = nullptr,
-                                  int cnt128 = 0) {
+                                  const void* ysp, hipStream_t s, const int32_t* klist = nullptr,
+                                  int kcnt = 0) {
   if (prec == LETKF_PREC_BF16X3_TRI32) prec = LETKF_PREC_BF16X3;  // the Gram of the mode
   if (prec == LETKF_PREC_FP64 || kp.K > 128)
     return launch_gram_fp64<T>(kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec, dual, s);
@@ -337,7 +337,7 @@ static hipError_t launch_gram_any(int prec, const KParams& kp, const Tables<T>& 
     return launch_gram_mixed<T>(prec == LETKF_PREC_FP32_DUAL64 ? LETKF_PREC_FP32 : prec, kp, tb, yT, act_p, loff,
                                 lbase, lobs, lrloc, a0, na, G, bvec, dual,
                                 prec == LETKF_PREC_BF16X3 ? ysp : nullptr, s,
-                                prec == LETKF_PREC_BF16X3 ? list128 : nullptr, cnt128);
+                                prec == LETKF_PREC_BF16X3 ? klist : nullptr, kcnt);
   hipError_t e = launch_gram_mixed<T>(LETKF_PREC_FP32, kp, tb, yT, act_p, loff, lbase, lobs, lrloc, a0, na, G, bvec,
                                       2, nullptr, s);
   if (e != hipSuccess) return e;
@@ -661,13 +661,14 @@ static int enqueue_device(letkf_handle* h, const letkf_grid_t* g, const letkf_ob
     }
     HIP_TRY(mark(e0 + 1));
     if (na > 0) {
-      // with the size buckets planned, the block Gram visits the 128 bucket's slots only
-      const size_t q128 = (size_t)c * kNQ + 4;
-      const bool bl = reg_eig && dual && K > 64 && q128 < pl.hist.size();
+      // with the size buckets planned, the K x K Gram kernels visit the
+      // slots of the bucket that holds size K only
+      const size_t qk = (size_t)c * kNQ + (K <= 64 ? (K - 1) / 16 : 4);
+      const bool bl = reg_eig && qk < pl.hist.size();
       HIP_TRY(launch_gram_any<T>(p.precision, kp, tb, h->yT.as<double>(), h->act_p.as<int32_t>(),
                                  h->loff.as<int64_t>(), lbase, h->lobs.as<int32_t>(), h->lrloc.as<double>(), a0, na,
                                  h->G.as<double>(), h->bvec.as<double>(), dual, ysp, s,
-                                 bl ? h->blist.as<int32_t>() + pl.boff[q128] : nullptr, bl ? pl.hist[q128] : 0));
+                                 bl ? h->blist.as<int32_t>() + pl.boff[qk] : nullptr, bl ? pl.hist[qk] : 0));
     }
     HIP_TRY(mark(e0 + 2));
     PointMajor<T> pm{nullptr, nullptr, p0, pl.pm_ld};

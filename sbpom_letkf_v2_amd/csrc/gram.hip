@@ -893,7 +893,8 @@ hipError_t launch_split_obs(int nobs, int K, const double* yT, void* ysp, hipStr
 // Y^T diag(w) dep sums in FP64.
 template <typename T>
 __global__ void __launch_bounds__(256) k_gram_wave_kk(int K, double diag, ObsSource<T> src, int na,
-                                                      double* __restrict__ G, double* __restrict__ bvec, int dual) {
+                                                      double* __restrict__ G, double* __restrict__ bvec, int dual,
+                                                      const int32_t* __restrict__ list) {
   struct RowMeta {
     double sw, swd;
   };
@@ -901,8 +902,10 @@ __global__ void __launch_bounds__(256) k_gram_wave_kk(int K, double diag, ObsSou
   __shared__ int mio[4][64];
   const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int b = blockIdx.x * 4 + w;
-  if (b >= na) return;
+  // list (optional): the na slots to visit, else slots 0 .. na - 1
+  const int i = blockIdx.x * 4 + w;
+  if (i >= na) return;
+  const int b = list ? list[i] : i;
   const int n = src.begin(b);
   if (dual && n < K) return;  // dual form: k_gram_wave
   const bool two = K > 32;
@@ -1009,7 +1012,7 @@ template <typename T>
 hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, const double* yT,
                              const int32_t* act_p, const int64_t* loff, long long lbase, const int32_t* lobs,
                              const double* lrloc, int a0, int na, double* G, double* bvec, int dual,
-                             const void* ysp, hipStream_t s, const int32_t* list128, int cnt128) {
+                             const void* ysp, hipStream_t s, const int32_t* klist, int kcnt) {
   if (na <= 0) return hipSuccess;
   if (kp.K > 128) return hipErrorInvalidValue;
   ObsSource<T> src{tb, yT, act_p, loff, lbase, lobs, lrloc, a0, 0, static_cast<const mf_bf8*>(ysp)};
@@ -1022,16 +1025,20 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb, c
       hipLaunchKernelGGL((k_gram_wave<T, true>), dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
     else if (dual)
       hipLaunchKernelGGL((k_gram_wave<T, false>), dim3((na + 3) / 4), dim3(256), 0, s, kp.K, src, na, G, bvec);
+    // the K x K slots lie in the size bucket of K: over its list when given
     if (kp.K <= 64) {
-      hipLaunchKernelGGL(k_gram_wave_kk<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, diag, src, na, G, bvec,
-                         dual);
+      if (!klist)
+        hipLaunchKernelGGL(k_gram_wave_kk<T>, dim3((na + 3) / 4), dim3(256), 0, s, kp.K, diag, src, na, G, bvec,
+                           dual, nullptr);
+      else if (kcnt > 0)
+        hipLaunchKernelGGL(k_gram_wave_kk<T>, dim3((kcnt + 3) / 4), dim3(256), 0, s, kp.K, diag, src, kcnt, G,
+                           bvec, dual, klist);
       return hipGetLastError();
     }
     if (dual) dual = 3;
-    // the rest are the slots of the 128 bucket: launched over its list when given
-    if (dual && list128) {
-      if (cnt128 > 0)
-        launch_gram_mixed_p<LETKF_PREC_BF16X3, T>(kp.K, diag, src, cnt128, G, bvec, dual, s, list128);
+    // the rest are the slots of the 128 bucket (dual 65 .. K - 1 and K x K)
+    if (dual && klist) {
+      if (kcnt > 0) launch_gram_mixed_p<LETKF_PREC_BF16X3, T>(kp.K, diag, src, kcnt, G, bvec, dual, s, klist);
       return hipGetLastError();
     }
   }

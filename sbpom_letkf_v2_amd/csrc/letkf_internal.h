@@ -87,9 +87,10 @@ hipError_t launch_gram_fp64(const KParams& kp, const Tables<T>& tb,
 // panel into FP32 totals for LETKF_PREC_FP32_PANEL), b in
 // FP64; K <= 128.
 // ysp (optional, BF16X3 only): the obs rows pre-split by launch_split_obs,
-// which the dual-form points' Gram then reads instead of yT.  list128 /
-// cnt128 (optional, BF16X3 with nmem > 64 and the dual form): the slots of
-// the 128 size bucket, the only ones the block kernel then has to visit.
+// which the dual-form points' Gram then reads instead of yT.  klist / kcnt
+// (optional, BF16X3): the slots of the size bucket that holds the K x K
+// problems, the only ones the K x K kernel (nmem <= 64) or, in the dual
+// form, the block kernel (nmem > 64) then has to visit.
 template <typename T>
 hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb,
                              const double* yT, const int32_t* act_p,
@@ -97,7 +98,7 @@ hipError_t launch_gram_mixed(int prec, const KParams& kp, const Tables<T>& tb,
                              const int32_t* lobs, const double* lrloc, int a0,
                              int na, double* G, double* bvec, int dual,
                              const void* ysp, hipStream_t s,
-                             const int32_t* list128 = nullptr, int cnt128 = 0);
+                             const int32_t* klist = nullptr, int kcnt = 0);
 
 // Obs rows of yT (nobs x K) as BF16 hi / lo pairs, 8 members a group
 // (zero past K), split_obs_groups(K) groups a row: 32 bytes a group, hi
