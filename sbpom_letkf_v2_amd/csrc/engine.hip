@@ -254,9 +254,10 @@ __device__ __forceinline__ int size_bucket(int m) { return (m - 1) / 16 < 4 ? (m
 // Per-(chunk, bucket) slot counts: each block counts 256 consecutive points
 // (one chunk: PC is a multiple of 256) with wave ballots into LDS and adds
 // one total per bucket (a global atomic per point contended on the few
-// bins: 45 ms at 10^7 points).
+// bins: 45 ms at 10^7 points); bblk[t * kNQ + q] keeps the count of
+// 256-point block t in bucket q for the ordered scatter.
 __global__ void __launch_bounds__(256) k_bucket_count(long long P, const int32_t* nloc, int K, int dual,
-                                                      long long PC, int32_t* hist) {
+                                                      long long PC, int32_t* hist, int32_t* bblk) {
   __shared__ int cnt[kNQ];
   const int lane = threadIdx.x & 63;
   for (long long p0 = blockIdx.x * 256LL; p0 < P; p0 += 256LL * gridDim.x) {
@@ -274,18 +275,51 @@ __global__ void __launch_bounds__(256) k_bucket_count(long long P, const int32_t
       if (lane == 0 && m) atomicAdd(&cnt[b], __popcll(m));
     }
     __syncthreads();
-    if (threadIdx.x < kNQ && cnt[threadIdx.x]) atomicAdd(&hist[(p0 / PC) * kNQ + threadIdx.x], cnt[threadIdx.x]);
+    if (threadIdx.x < kNQ) {
+      bblk[(p0 / 256) * kNQ + threadIdx.x] = cnt[threadIdx.x];
+      if (cnt[threadIdx.x]) atomicAdd(&hist[(p0 / PC) * kNQ + threadIdx.x], cnt[threadIdx.x]);
+    }
     __syncthreads();
   }
 }
 
-// Stable-within-block scatter of active points into per-(chunk, bucket)
-// lists: each block takes 256 consecutive points, ranks them per bucket with
-// wave ballots and reserves one range per bucket, so lists stay close to
-// point order (neighbouring points then share cache lines in the update).
+// The block counts of each (chunk, bucket) turned into exclusive prefix
+// sums in point order, one workgroup per (chunk, bucket): the offset of
+// block t's slots within the chunk's list of that bucket.
+__global__ void __launch_bounds__(256) k_bucket_blockscan(long long nblk, long long PC, int32_t* bblk) {
+  __shared__ int wsum[4];
+  const int c = blockIdx.x / kNQ, q = blockIdx.x % kNQ;
+  const long long t0 = (long long)c * PC / 256;
+  const long long t1 = min(((long long)(c + 1) * PC + 255) / 256, nblk);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int carry = 0;
+  for (long long b0 = t0; b0 < t1; b0 += 256) {
+    const long long t = b0 + threadIdx.x;
+    const int v = t < t1 ? bblk[t * kNQ + q] : 0;
+    int x = v;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int pre = carry;
+    for (int u = 0; u < w; ++u) pre += wsum[u];
+    if (t < t1) bblk[t * kNQ + q] = pre + x - v;
+    carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+}
+
+// Scatter of active points into per-(chunk, bucket) lists in point order:
+// each block takes 256 consecutive points, ranks them per bucket with wave
+// ballots and places them at the list start of (chunk, bucket) plus the
+// block's prefix offset (bblk after k_bucket_blockscan), so neighbouring
+// points stay together in every bucket's kernels (shared obs rows in L2).
 __global__ void __launch_bounds__(256) k_bucket_scatter(long long P, const int32_t* nloc, const int32_t* actidx,
-                                                        int K, int dual, long long PC, int32_t* cursor,
-                                                        int32_t* list) {
+                                                        int K, int dual, long long PC, const int32_t* cursor,
+                                                        const int32_t* bblk, int32_t* list) {
   __shared__ int wcnt[4][kNQ];
   __shared__ int base[kNQ];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -308,8 +342,7 @@ __global__ void __launch_bounds__(256) k_bucket_scatter(long long P, const int32
     __syncthreads();
     if (threadIdx.x < kNQ) {
       const int b = threadIdx.x;
-      const int tot = wcnt[0][b] + wcnt[1][b] + wcnt[2][b] + wcnt[3][b];
-      base[b] = tot ? atomicAdd(&cursor[c * kNQ + b], tot) : 0;
+      base[b] = cursor[c * kNQ + b] + bblk[(p0 / 256) * kNQ + b];
     }
     __syncthreads();
     if (q >= 0) {
@@ -507,9 +540,13 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     HIP_TRY(h->bhist.ensure(sizeof(int32_t) * nh));
     HIP_TRY(h->bcur.ensure(sizeof(int32_t) * nh));
     HIP_TRY(h->blist.ensure(sizeof(int32_t) * (size_t)(st.active_points + 1)));
+    const long long nblk = (P + 255) / 256;
+    HIP_TRY(h->bblk.ensure(sizeof(int32_t) * (size_t)nblk * kNQ));
     HIP_TRY(hipMemsetAsync(h->bhist.ptr, 0, sizeof(int32_t) * nh, s));
     hipLaunchKernelGGL(k_bucket_count, dim3(2048), dim3(256), 0, s, P, h->nloc.as<int32_t>(), K, dual, PC,
-                       h->bhist.as<int32_t>());
+                       h->bhist.as<int32_t>(), h->bblk.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_bucket_blockscan, dim3((unsigned)nh), dim3(256), 0, s, nblk, PC, h->bblk.as<int32_t>());
     HIP_TRY(hipGetLastError());
     hist.resize(nh);
     boff.resize(nh);
@@ -523,7 +560,8 @@ static int plan_device(letkf_handle* h, const letkf_grid_t* g, const letkf_obs_t
     }
     HIP_TRY(hipMemcpyAsync(h->bcur.ptr, boff.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_bucket_scatter, dim3(2048), dim3(256), 0, s, P, h->nloc.as<int32_t>(),
-                       h->actidx.as<int32_t>(), K, dual, PC, h->bcur.as<int32_t>(), h->blist.as<int32_t>());
+                       h->actidx.as<int32_t>(), K, dual, PC, h->bcur.as<int32_t>(), h->bblk.as<int32_t>(),
+                       h->blist.as<int32_t>());
     HIP_TRY(hipGetLastError());
     // the host copy of boff must stay alive until the copy has run
     HIP_TRY(hipStreamSynchronize(s));

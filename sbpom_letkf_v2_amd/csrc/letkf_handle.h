@@ -99,6 +99,7 @@ struct letkf_handle {
   DevBuf adj_part;  // adjoint: per-point (dL/dalpha, dL/drho, dL/dsigma_obs, dL/dsigma_obsv), block sums
   DevBuf adj_pair, adj_sort;  // adjoint: per-pair terms; sort keys, values and scratch
   DevBuf bhist, bcur, blist;              // per-chunk size buckets of active slots
+  DevBuf bblk;                            // per-256-point-block bucket counts / offsets
   DevBuf xpm, apm;                        // point-major forecast / analysis of a chunk (register update)
   DevBuf pst;                             // their forecast (mean, spread) per point and variable
   DevBuf ysp;                             // obs rows as BF16 hi / lo groups (dual-form BF16x3 Gram)
@@ -128,7 +129,7 @@ struct letkf_handle {
   }
   void release_all() {
     DevBuf* all[] = {&nloc, &loff, &actidx, &act_p, &scan_tmp, &yT, &lobs, &lrloc, &G, &lam, &V, &bvec,
-                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &ratt, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &xpm, &apm, &pst, &ysp, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
+                     &status, &bounds, &nrow, &bin_tmp, &eD, &eE, &logc, &logs, &logi, &logn, &logx, &logr, &yvec, &rotsum, &otrig, &ratt, &adj_part, &adj_pair, &adj_sort, &bhist, &bcur, &blist, &bblk, &xpm, &apm, &pst, &ysp, &h_grid, &h_obs, &h_f3, &h_f2, &h_a3, &h_a2};
     for (DevBuf* b : all) b->release();
     ratt_ok = false;
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
